@@ -1,0 +1,330 @@
+// api.cpp -- the C ABI declared in include/svo/svo.h.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "svo_host.h"
+
+using namespace svo;
+
+namespace {
+
+struct FieldRange {
+  const void* dev = nullptr;
+  uint64_t elem_bytes = 0;
+  uint64_t count = 0;
+  bool is_code = false;
+};
+
+FieldRange field_range(const svo_tree* t, int field) {
+  FieldRange r;
+  const BatchOutput& o = *t->out;
+  const NodeArrays& n = o.nodes;
+  const uint64_t N = t->info.n_nodes, b = t->node_base;
+  const uint64_t cb = (uint64_t)o.code_bytes;
+  switch (field) {
+    case SVO_FIELD_VOXELS:
+      r.dev = (const char*)o.voxels + cb * t->vox_base;
+      r.elem_bytes = cb;
+      r.count = t->info.n_voxels;
+      r.is_code = true;
+      break;
+    case SVO_FIELD_CODE:
+      r.dev = (const char*)n.code + cb * b;
+      r.elem_bytes = cb;
+      r.count = N;
+      r.is_code = true;
+      break;
+    case SVO_FIELD_ISLEAF: r.dev = n.is_leaf + b; r.elem_bytes = 1; r.count = N; break;
+    case SVO_FIELD_ORIGIN: r.dev = n.origin + 3 * b; r.elem_bytes = 12; r.count = N; break;
+    case SVO_FIELD_WIDTH: r.dev = n.width + b; r.elem_bytes = 4; r.count = N; break;
+    case SVO_FIELD_PARENT: r.dev = n.parent + b; r.elem_bytes = 4; r.count = N; break;
+    case SVO_FIELD_CHILDS: r.dev = n.childs + 8 * b; r.elem_bytes = 32; r.count = N; break;
+    case SVO_FIELD_NEIGHBORS: r.dev = n.neighbors + 27 * b; r.elem_bytes = 108; r.count = N; break;
+    case SVO_FIELD_VERTICES:
+      r.dev = o.verts + t->v_base; r.elem_bytes = 16; r.count = t->info.n_vertices; break;
+    case SVO_FIELD_EDGES:
+      r.dev = o.edges + t->e_base; r.elem_bytes = 28; r.count = t->info.n_edges; break;
+    case SVO_FIELD_FACES:
+      r.dev = o.faces + t->f_base; r.elem_bytes = 28; r.count = t->info.n_faces; break;
+    default: break;
+  }
+  return r;
+}
+
+int copy_d2h(Context& C, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return SVO_OK;
+  std::lock_guard<std::mutex> lk(C.aux_mu);
+  hipSetDevice(C.device);
+  if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, C.aux_stream) != hipSuccess)
+    return SVO_E_HIP;
+  if (hipStreamSynchronize(C.aux_stream) != hipSuccess) return SVO_E_HIP;
+  return SVO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* svo_status_str(int s) {
+  switch (s) {
+    case SVO_OK: return "ok";
+    case SVO_E_INVALID: return "invalid argument or input";
+    case SVO_E_RES_LIMIT: return "resolution beyond the key width";
+    case SVO_E_OOM: return "out of device memory or budget";
+    case SVO_E_HIP: return "HIP runtime error";
+    case SVO_E_IO: return "file cannot be opened or written";
+    case SVO_E_PARSE: return "OBJ line of 255 or more characters";
+    case SVO_E_EMPTY: return "empty mesh or no voxel";
+    case SVO_E_BUSY: return "server queue full";
+    case SVO_E_RANGE: return "destination buffer too small";
+    default: return "unknown status";
+  }
+}
+
+int svo_ctx_create(int device, uint64_t mem_budget, svo_ctx** out) {
+  if (!out) return SVO_E_INVALID;
+  *out = nullptr;
+  svo_ctx* c = new svo_ctx();
+  const int st = c->impl.init(device, mem_budget);
+  if (st != SVO_OK) {
+    delete c;
+    return st;
+  }
+  *out = c;
+  return SVO_OK;
+}
+
+int svo_ctx_destroy(svo_ctx* ctx) {
+  if (!ctx) return SVO_E_INVALID;
+  ctx->impl.destroy();
+  delete ctx;
+  return SVO_OK;
+}
+
+int svo_ctx_set_timing(svo_ctx* ctx, int enable) {
+  if (!ctx) return SVO_E_INVALID;
+  ctx->impl.timing = enable != 0;
+  return SVO_OK;
+}
+
+int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap, int* n) {
+  if (!ctx || !n) return SVO_E_INVALID;
+  Context& C = ctx->impl;
+  std::lock_guard<std::mutex> lk(C.stage_mu);
+  const int k = (int)C.stage_ms.size();
+  *n = k;
+  for (int i = 0; i < k && i < cap; ++i) {
+    if (ms) ms[i] = C.stage_ms[i];
+    if (names) names[i] = C.stage_names[i].c_str();
+  }
+  return k > cap ? SVO_E_RANGE : SVO_OK;
+}
+
+uint64_t svo_ctx_bytes_in_use(const svo_ctx* ctx) { return ctx ? ctx->impl.in_use.load() : 0; }
+
+int svo_load_obj(const char* path, svo_mesh** out) {
+  if (!path || !out) return SVO_E_INVALID;
+  *out = nullptr;
+  svo_mesh* m = new svo_mesh();
+  const int st = load_obj_file(path, m->points, m->tris);
+  if (st != SVO_OK) {
+    delete m;
+    return st;
+  }
+  *out = m;
+  return SVO_OK;
+}
+
+int svo_mesh_view(const svo_mesh* mesh, svo_mesh_desc* out) {
+  if (!mesh || !out) return SVO_E_INVALID;
+  out->xyz = mesh->points.data();
+  out->n_points = mesh->points.size() / 3;
+  out->tri = mesh->tris.data();
+  out->n_tris = mesh->tris.size() / 3;
+  return SVO_OK;
+}
+
+void svo_mesh_free(svo_mesh* mesh) { delete mesh; }
+
+int svo_validate(const svo_mesh_desc* mesh, const svo_params* params) {
+  return validate_params(mesh, params, nullptr);
+}
+
+uint64_t svo_estimate_bytes(uint64_t n_points, uint64_t n_tris, const svo_params* params) {
+  return params ? estimate_bytes(n_points, n_tris, params) : 0;
+}
+
+static int build_common(svo_ctx* ctx, const svo_mesh_desc* mesh, const svo_params* params,
+                        int on_device, svo_tree** out) {
+  if (!ctx || !mesh || !params || !out) return SVO_E_INVALID;
+  *out = nullptr;
+  std::vector<MeshInput> in(1);
+  in[0].desc = *mesh;
+  in[0].params = *params;
+  in[0].on_device = on_device != 0;
+  std::vector<svo_tree*> trees;
+  std::vector<int> st;
+  const int s = run_batch(ctx->impl, in, trees, st);
+  if (s != SVO_OK) return s;
+  if (st[0] != SVO_OK) return st[0];
+  *out = trees[0];
+  return SVO_OK;
+}
+
+int svo_build(svo_ctx* ctx, const svo_mesh_desc* mesh, const svo_params* params, svo_tree** out) {
+  return build_common(ctx, mesh, params, 0, out);
+}
+
+int svo_build_device(svo_ctx* ctx, const svo_mesh_desc* mesh, const svo_params* params,
+                     svo_tree** out) {
+  return build_common(ctx, mesh, params, 1, out);
+}
+
+int svo_build_batch(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params* params,
+                    uint32_t n, int meshes_on_device, svo_tree** trees, int* statuses) {
+  if (!ctx || (n && (!meshes || !params || !trees))) return SVO_E_INVALID;
+  std::vector<MeshInput> in(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    in[i].desc = meshes[i];
+    in[i].params = params[i];
+    in[i].on_device = meshes_on_device != 0;
+  }
+  std::vector<svo_tree*> tr;
+  std::vector<int> st;
+  const int s = run_batch(ctx->impl, in, tr, st);
+  int agg = s;
+  for (uint32_t i = 0; i < n; ++i) {
+    trees[i] = (s == SVO_OK) ? tr[i] : nullptr;
+    if (statuses) statuses[i] = (s == SVO_OK) ? st[i] : s;
+    if (agg == SVO_OK && st[i] != SVO_OK) agg = st[i];
+  }
+  return agg;
+}
+
+int svo_tree_info_get(const svo_tree* tree, svo_tree_info* out) {
+  if (!tree || !out) return SVO_E_INVALID;
+  *out = tree->info;
+  return SVO_OK;
+}
+
+uint64_t svo_tree_field_bytes(const svo_tree* tree, int field) {
+  if (!tree) return 0;
+  const uint64_t kb = tree->key_bits_out == 64 ? 8 : 4;
+  const svo_tree_info& I = tree->info;
+  switch (field) {
+    case SVO_FIELD_VOXELS: return kb * I.n_voxels;
+    case SVO_FIELD_CODE: return kb * I.n_nodes;
+    case SVO_FIELD_LEVEL_NODES: return 8ull * (uint64_t)I.depth;
+    case SVO_FIELD_NODES_AOS: return 168ull * I.n_nodes;
+    default: {
+      const FieldRange r = field_range(tree, field);
+      return r.elem_bytes * r.count;
+    }
+  }
+}
+
+const void* svo_tree_device_ptr(const svo_tree* tree, int field) {
+  if (!tree) return nullptr;
+  return field_range(tree, field).dev;
+}
+
+int svo_tree_copy_field(const svo_tree* tree, int field, void* dst, uint64_t dst_bytes) {
+  if (!tree || (!dst && svo_tree_field_bytes(tree, field))) return SVO_E_INVALID;
+  if (field < 0 || field >= SVO_FIELD_COUNT) return SVO_E_INVALID;
+  const uint64_t need = svo_tree_field_bytes(tree, field);
+  if (dst_bytes < need) return SVO_E_RANGE;
+  Context& C = *tree->ctx;
+  if (field == SVO_FIELD_LEVEL_NODES) {
+    std::memcpy(dst, tree->info.level_nodes, need);
+    return SVO_OK;
+  }
+  if (field == SVO_FIELD_NODES_AOS) {
+    // repack on the device in chunks, then copy
+    const uint64_t N = tree->info.n_nodes;
+    const uint64_t chunk = std::min<uint64_t>(N, 1ull << 20);
+    if (!N) return SVO_OK;
+    void* tmp = nullptr;
+    int st = C.dev_malloc(&tmp, 168 * chunk);
+    if (st != SVO_OK) return st;
+    for (uint64_t a = 0; a < N && st == SVO_OK; a += chunk) {
+      const uint64_t k = std::min(chunk, N - a);
+      {
+        std::lock_guard<std::mutex> lk(C.aux_mu);
+        hipSetDevice(C.device);
+        launch_pack_aos(tree->out->nodes, tree->node_base + a, k, tmp, C.aux_stream);
+      }
+      st = copy_d2h(C, (char*)dst + 168 * a, tmp, 168 * k);
+    }
+    C.dev_free(tmp, 168 * chunk);
+    return st;
+  }
+  const FieldRange r = field_range(tree, field);
+  if (!r.is_code || (uint64_t)tree->out->code_bytes * 8 == (uint64_t)tree->key_bits_out)
+    return copy_d2h(C, dst, r.dev, r.elem_bytes * r.count);
+  // stored width differs from the requested one: convert on the host
+  std::vector<uint8_t> tmp(r.elem_bytes * r.count);
+  int st = copy_d2h(C, tmp.data(), r.dev, tmp.size());
+  if (st != SVO_OK) return st;
+  for (uint64_t i = 0; i < r.count; ++i) {
+    uint64_t v = 0;
+    std::memcpy(&v, tmp.data() + r.elem_bytes * i, r.elem_bytes);
+    if (tree->key_bits_out == 64) ((uint64_t*)dst)[i] = v;
+    else ((uint32_t*)dst)[i] = (uint32_t)v;
+  }
+  return SVO_OK;
+}
+
+int svo_write_obj(const svo_tree* tree, const char* dir, const char* stem, char* path_out,
+                  size_t path_cap) {
+  if (!tree || !stem) return SVO_E_INVALID;
+  // base_filename + "_" + treeDepth + "_tree.obj", SVO.cu:738-739
+  std::string path = (dir && *dir) ? std::string(dir) + "/" : std::string();
+  path += std::string(stem) + "_" + std::to_string(tree->info.depth) + "_tree.obj";
+  if (path_out && path_cap) {
+    std::strncpy(path_out, path.c_str(), path_cap - 1);
+    path_out[path_cap - 1] = 0;
+  }
+  const uint64_t N = tree->info.n_nodes;
+  std::vector<float> org(3 * N), w(N);
+  int st = svo_tree_copy_field(tree, SVO_FIELD_ORIGIN, org.data(), 12 * N);
+  if (st == SVO_OK) st = svo_tree_copy_field(tree, SVO_FIELD_WIDTH, w.data(), 4 * N);
+  if (st != SVO_OK) return st;
+  return write_tree_obj(path.c_str(), org.data(), w.data(), N);
+}
+
+int svo_query_neighbors(const svo_tree* tree, const uint32_t* node, const uint8_t* slot,
+                        uint64_t n, uint32_t* out) {
+  if (!tree || (n && (!node || !slot || !out))) return SVO_E_INVALID;
+  if (!n) return SVO_OK;
+  Context& C = *tree->ctx;
+  const size_t bytes = 4 * n + 4 * n + n + 512;
+  void* tmp = nullptr;
+  int st = C.dev_malloc(&tmp, bytes);
+  if (st != SVO_OK) return st;
+  uint32_t* d_node = (uint32_t*)tmp;
+  uint32_t* d_out = d_node + n;
+  uint8_t* d_slot = (uint8_t*)(d_out + n);
+  {
+    std::lock_guard<std::mutex> lk(C.aux_mu);
+    hipSetDevice(C.device);
+    hipStream_t s = C.aux_stream;
+    if (hipMemcpyAsync(d_node, node, 4 * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_slot, slot, n, hipMemcpyHostToDevice, s) != hipSuccess) {
+      st = SVO_E_HIP;
+    } else {
+      launch_query_neighbors(tree->out->nodes.neighbors + 27 * tree->node_base,
+                             tree->info.n_nodes, d_node, d_slot, n, d_out, s);
+      if (hipMemcpyAsync(out, d_out, 4 * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        st = SVO_E_HIP;
+    }
+  }
+  C.dev_free(tmp, bytes);
+  return st;
+}
+
+void svo_tree_free(svo_tree* tree) { delete tree; }
+
+}  // extern "C"

@@ -1,0 +1,515 @@
+// k_voxelize.hip -- point bbox, grid parameters and surface voxelization.
+//
+// Follows the reference's surfaceVoxelize (SVO.cu:37-188): per triangle a
+// plane test and three 2-D projection tests over every cell of the clamped
+// bounding box, with the same float operation order (built with
+// -ffp-contract=off, correctly rounded divide and sqrt, denormals kept).
+//
+// Work distribution (the reference gives one thread per triangle):
+//  * k_classify   one lane per triangle: box cells -> key capacity and the
+//                 number of big-triangle chunks (exact, read back once).
+//  * k_vox_small  one lane per triangle whose box has <= small_max cells;
+//                 lanes walk their box row by row, and the yz-projection
+//                 test, which does not depend on x, is evaluated once per
+//                 row (an exact hoist: the same operands, the same rounding).
+//                 Larger triangles are cut into row chunks and queued.
+//  * k_vox_big    one wave64 per chunk: the row terms are wave-uniform and
+//                 the 64 lanes cover consecutive x.
+// Hits are staged per wave in LDS and flushed with one atomic per ~1K keys.
+#include "svo_internal.h"
+
+namespace svo {
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int WAVES = BLOCK / 64;
+constexpr int STAGE = 1024;        // keys staged per wave
+constexpr int BBOX_CHUNK = 4096;   // points per bbox block
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline uint64_t bcast_first(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline uint32_t find_mesh(const uint64_t* base, uint32_t n_meshes, uint64_t gid) {
+  uint32_t lo = 0, hi = n_meshes;  // base[lo] <= gid < base[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (base[mid] <= gid) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// ------------------------------------------------------------------ bbox
+__global__ void k_bbox_init(uint32_t* acc, uint32_t n_meshes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_meshes * 8) return;
+  const uint32_t f = i & 7;
+  acc[i] = f < 3 ? 0xFFFFFFFFu : 0u;  // ordered min starts at +max, max at 0
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bbox(const MeshDev* meshes, uint32_t* acc) {
+  const MeshDev md = meshes[blockIdx.y];
+  const uint64_t start = (uint64_t)blockIdx.x * BBOX_CHUNK;
+  if (start >= md.n_points) return;
+  const uint64_t end = start + BBOX_CHUNK < md.n_points ? start + BBOX_CHUNK : md.n_points;
+  uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0, 0, 0};
+  uint32_t bad = 0;
+  for (uint64_t i = start + threadIdx.x; i < end; i += BLOCK) {
+    for (int a = 0; a < 3; ++a) {
+      const float v = md.pts[3 * i + a];
+      if (!isfinite(v)) {
+        bad = 1;
+        continue;
+      }
+      const uint32_t k = float_to_ordered(v);
+      mn[a] = k < mn[a] ? k : mn[a];
+      mx[a] = k > mx[a] ? k : mx[a];
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    for (int a = 0; a < 3; ++a) {
+      const uint32_t omn = __shfl_xor(mn[a], off);
+      const uint32_t omx = __shfl_xor(mx[a], off);
+      mn[a] = omn < mn[a] ? omn : mn[a];
+      mx[a] = omx > mx[a] ? omx : mx[a];
+    }
+    bad |= __shfl_xor(bad, off);
+  }
+  __shared__ uint32_t red[WAVES][7];
+  const uint32_t w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    for (int a = 0; a < 3; ++a) {
+      red[w][a] = mn[a];
+      red[w][3 + a] = mx[a];
+    }
+    red[w][6] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    uint32_t v = red[0][threadIdx.x];
+    for (int k = 1; k < WAVES; ++k) {
+      const uint32_t o = red[k][threadIdx.x];
+      if (threadIdx.x < 3) v = o < v ? o : v;
+      else if (threadIdx.x < 6) v = o > v ? o : v;
+      else v |= o;
+    }
+    uint32_t* a = acc + 8 * blockIdx.y;
+    if (threadIdx.x < 3) atomicMin(a + threadIdx.x, v);
+    else if (threadIdx.x < 6) atomicMax(a + threadIdx.x, v);
+    else if (v) atomicOr(a + 6, 1u);
+  }
+}
+
+// createMeshBBCube (BaseModel.h:24-49) and the voxel size (SVO.cu:348-350).
+__global__ void k_params(const MeshDev* meshes, uint32_t n_meshes, const uint32_t* acc,
+                         MeshParams* params) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_meshes) return;
+  const uint32_t* a = acc + 8 * m;
+  MeshParams P;
+  P.status = a[6] ? ERR_NONFINITE : 0u;
+  float mn[3], mx[3], len[3], amin[3], amax[3];
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = ordered_to_float(a[k]);
+    mx[k] = ordered_to_float(a[3 + k]);
+    len[k] = mx[k] - mn[k];
+    amin[k] = mn[k];
+    amax[k] = mx[k];
+    P.res[k] = meshes[m].res[k];
+  }
+  const float max_length = fmax_r(len[0], fmax_r(len[1], len[2]));
+  if (!(max_length > 0.0f)) P.status |= ERR_DEGENERATE;
+  for (int k = 0; k < 3; ++k) {
+    if (max_length == len[k]) continue;
+    const float delta = max_length - len[k];
+    amin[k] = mn[k] - (delta / 2.0f);
+    amax[k] = mx[k] + (delta / 2.0f);
+  }
+  for (int k = 0; k < 3; ++k) {
+    const float eps = (amax[k] - amin[k]) / 10001.0f;
+    amin[k] -= eps;
+    amax[k] += eps;
+  }
+  for (int k = 0; k < 3; ++k) {
+    P.bmin[k] = amin[k];
+    P.bmax[k] = amax[k];
+    P.width[k] = amax[k] - amin[k];
+    P.origin[k] = amin[k];
+    P.unit[k] = P.width[k] / (float)P.res[k];
+  }
+  params[m] = P;
+}
+
+// ------------------------------------------------------------------ triangle setup
+struct TriSetup {
+  float n[3];
+  float d1, d2;
+  float ne[3][3][2];  // [plane xy|yz|zx][edge][2]
+  float de[3][3];
+  float u[3];
+  int g0[3], g1[3];
+};
+
+// Loads a triangle relative to gridOrigin (SVO.cu:58-63).  false on a bad index.
+__device__ inline bool load_tri(const MeshDev& md, const MeshParams& P, uint64_t local,
+                                float v[3][3]) {
+  const int32_t* t = md.tri + 3 * local;
+  for (int k = 0; k < 3; ++k) {
+    const int32_t i = t[k];
+    if (i < 0 || (uint64_t)i >= md.n_points) return false;
+    for (int a = 0; a < 3; ++a) v[k][a] = md.pts[3 * (uint64_t)i + a] - P.origin[a];
+  }
+  return true;
+}
+
+// Triangle box in grid cells (SVO.cu:73-85).
+__device__ inline void tri_box(const float v[3][3], const MeshParams& P, int g0[3], int g1[3]) {
+  for (int a = 0; a < 3; ++a) {
+    const float lo = fmin_r(v[0][a], fmin_r(v[1][a], v[2][a]));
+    const float hi = fmax_r(v[0][a], fmax_r(v[1][a], v[2][a]));
+    g0[a] = clamp_i((int)(lo / P.unit[a]), 0, P.res[a] - 1);
+    g1[a] = clamp_i((int)(hi / P.unit[a]), 0, P.res[a] - 1);
+  }
+}
+
+// Plane and projection test constants (SVO.cu:65-149).
+__device__ inline void tri_setup(const float v[3][3], const MeshParams& P, TriSetup& S) {
+  float e[3][3];
+  for (int a = 0; a < 3; ++a) {
+    e[0][a] = v[1][a] - v[0][a];
+    e[1][a] = v[2][a] - v[1][a];
+    e[2][a] = v[0][a] - v[2][a];
+    S.u[a] = P.unit[a];
+  }
+  float n0 = e[0][1] * e[1][2] - e[0][2] * e[1][1];
+  float n1 = e[0][2] * e[1][0] - e[0][0] * e[1][2];
+  float n2 = e[0][0] * e[1][1] - e[0][1] * e[1][0];
+  const float zz = n0 * n0 + (n1 * n1 + n2 * n2);  // squaredNorm, a0 + (a1 + a2)
+  if (zz > 0.0f) {
+    const float s = sqrtf(zz);
+    n0 = n0 / s;
+    n1 = n1 / s;
+    n2 = n2 / s;
+  }
+  S.n[0] = n0;
+  S.n[1] = n1;
+  S.n[2] = n2;
+  tri_box(v, P, S.g0, S.g1);
+  const float c0 = n0 > 0.0f ? S.u[0] : 0.0f;
+  const float c1 = n1 > 0.0f ? S.u[1] : 0.0f;
+  const float c2 = n2 > 0.0f ? S.u[2] : 0.0f;
+  S.d1 = n0 * (c0 - v[0][0]) + (n1 * (c1 - v[0][1]) + n2 * (c2 - v[0][2]));
+  S.d2 = n0 * ((S.u[0] - c0) - v[0][0]) +
+         (n1 * ((S.u[1] - c1) - v[0][1]) + n2 * ((S.u[2] - c2) - v[0][2]));
+  const bool flip[3] = {n2 < 0.0f, n0 < 0.0f, n1 < 0.0f};
+  // plane p uses axes (a, b): xy = (0, 1), yz = (1, 2), zx = (2, 0)
+  for (int p = 0; p < 3; ++p) {
+    const int a = p, b = (p + 1) % 3;
+    for (int i = 0; i < 3; ++i) {
+      float x0 = -1.0f * e[i][b];
+      float x1 = e[i][a];
+      if (flip[p]) {
+        x0 = -x0;
+        x1 = -x1;
+      }
+      S.ne[p][i][0] = x0;
+      S.ne[p][i][1] = x1;
+      S.de[p][i] = ((-1.0f * (x0 * v[i][a] + x1 * v[i][b])) + fmax_r(0.0f, S.u[a] * x0)) +
+                   fmax_r(0.0f, S.u[b] * x1);
+    }
+  }
+}
+
+// Row terms of cell row (y, z): everything of the 13 tests that does not
+// depend on x.  Returns false when a yz-projection test fails (the whole row
+// is empty).
+struct RowTerms {
+  float A;       // n.y*p.y + n.z*p.z  (inner sum of n.dot(p))
+  float rxy[3];  // ne_xy[i].y * p.y
+  float rzx[3];  // ne_zx[i].x * p.z
+};
+__device__ inline bool row_terms(const TriSetup& S, int y, int z, RowTerms& R) {
+  const float py = (float)y * S.u[1];
+  const float pz = (float)z * S.u[2];
+  bool ok = true;
+  for (int i = 0; i < 3; ++i)
+    ok = ok && !((S.ne[1][i][0] * py + S.ne[1][i][1] * pz) + S.de[1][i] < 0.0f);
+  R.A = S.n[1] * py + S.n[2] * pz;
+  for (int i = 0; i < 3; ++i) {
+    R.rxy[i] = S.ne[0][i][1] * py;
+    R.rzx[i] = S.ne[2][i][0] * pz;
+  }
+  return ok;
+}
+__device__ inline bool cell_hit(const TriSetup& S, const RowTerms& R, int x) {
+  const float px = (float)x * S.u[0];
+  const float t = S.n[0] * px + R.A;
+  if ((t + S.d1) * (t + S.d2) > 0.0f) return false;
+  bool ok = true;
+  for (int i = 0; i < 3; ++i) {
+    ok = ok && !((S.ne[0][i][0] * px + R.rxy[i]) + S.de[0][i] < 0.0f);
+    ok = ok && !((R.rzx[i] + S.ne[2][i][1] * px) + S.de[2][i] < 0.0f);
+  }
+  return ok;
+}
+
+template <typename K>
+struct Stager {
+  K* buf;            // this wave's LDS stage
+  uint32_t n = 0;    // wave-uniform count
+  K* keys;
+  unsigned long long* count;
+  __device__ inline void flush() {
+    wave_sync();
+    uint64_t base = 0;
+    if (__lane_id() == 0) base = atomicAdd(count, (unsigned long long)n);
+    base = bcast_first(base);
+    for (uint32_t i = __lane_id(); i < n; i += 64) keys[base + i] = buf[i];
+    wave_sync();
+    n = 0;
+  }
+  __device__ inline void push(bool hit, K key) {
+    const uint64_t mask = __ballot(hit);
+    if (mask == 0) return;
+    if (hit) buf[n + lanes_below(mask)] = key;
+    n += (uint32_t)__popcll(mask);
+    if (n > STAGE - 64) flush();
+  }
+  __device__ inline void finish() {
+    if (n) flush();
+  }
+};
+
+__device__ inline uint32_t rows_per_chunk(uint32_t chunk_cells, int nx) {
+  const uint32_t r = chunk_cells / (uint32_t)nx;
+  return r ? r : 1u;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
+                                                    const MeshParams* params,
+                                                    const uint64_t* tri_base, uint32_t n_meshes,
+                                                    uint64_t n_tris, uint32_t small_max,
+                                                    uint32_t chunk_cells,
+                                                    unsigned long long* totals) {
+  const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  uint64_t cells = 0, chunks = 0;
+  if (gid < n_tris) {
+    const uint32_t m = find_mesh(tri_base, n_meshes, gid);
+    const MeshDev& md = meshes[m];
+    const MeshParams& P = params[m];
+    float v[3][3];
+    if (P.status == 0 && load_tri(md, P, gid - md.tri_base, v)) {
+      int g0[3], g1[3];
+      tri_box(v, P, g0, g1);
+      const uint64_t nx = g1[0] - g0[0] + 1, ny = g1[1] - g0[1] + 1, nz = g1[2] - g0[2] + 1;
+      cells = nx * ny * nz;
+      if (cells > small_max) {
+        const uint64_t rpc = rows_per_chunk(chunk_cells, (int)nx);
+        chunks = (ny * nz + rpc - 1) / rpc;
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    cells += __shfl_xor(cells, off);
+    chunks += __shfl_xor(chunks, off);
+  }
+  __shared__ uint64_t red[WAVES][2];
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6][0] = cells;
+    red[threadIdx.x >> 6][1] = chunks;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint64_t s = 0;
+    for (int w = 0; w < WAVES; ++w) s += red[w][threadIdx.x];
+    if (s) atomicAdd(totals + threadIdx.x, (unsigned long long)s);
+  }
+}
+
+__global__ void k_flag_bad_index(const MeshDev* meshes, MeshParams* params,
+                                 const uint64_t* tri_base, uint32_t n_meshes, uint64_t n_tris) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n_tris) return;
+  const uint32_t m = find_mesh(tri_base, n_meshes, gid);
+  const MeshDev& md = meshes[m];
+  const int32_t* t = md.tri + 3 * (gid - md.tri_base);
+  for (int k = 0; k < 3; ++k)
+    if (t[k] < 0 || (uint64_t)t[k] >= md.n_points) atomicOr(&params[m].status, ERR_INDEX);
+}
+
+template <typename K>
+__global__ __launch_bounds__(BLOCK) void k_vox_small(
+    const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base, uint32_t n_meshes,
+    uint64_t n_tris, int cb, uint32_t small_max, uint32_t chunk_cells, K* keys,
+    unsigned long long* key_count, BigItem* big, unsigned long long* big_count, uint64_t big_cap) {
+  __shared__ K stage[WAVES][STAGE];
+  Stager<K> st;
+  st.buf = stage[threadIdx.x >> 6];
+  st.keys = keys;
+  st.count = key_count;
+  const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  TriSetup S;
+  uint64_t count = 0;
+  K mesh_key = 0;
+  if (gid < n_tris) {
+    const uint32_t m = find_mesh(tri_base, n_meshes, gid);
+    const MeshDev& md = meshes[m];
+    const MeshParams P = params[m];
+    float v[3][3];
+    if (P.status == 0 && load_tri(md, P, gid - md.tri_base, v)) {
+      tri_setup(v, P, S);
+      const uint64_t nx = S.g1[0] - S.g0[0] + 1, ny = S.g1[1] - S.g0[1] + 1,
+                     nz = S.g1[2] - S.g0[2] + 1;
+      const uint64_t cells = nx * ny * nz;
+      if (cells <= small_max) {
+        count = cells;
+      } else {
+        const uint64_t rpc = rows_per_chunk(chunk_cells, (int)nx);
+        const uint64_t nch = (ny * nz + rpc - 1) / rpc;
+        const uint64_t at = atomicAdd(big_count, (unsigned long long)nch);
+        for (uint64_t j = 0; j < nch; ++j)
+          if (at + j < big_cap) big[at + j] = BigItem{(uint32_t)gid, (uint32_t)(j * rpc)};
+      }
+      mesh_key = (K)m << cb;
+    }
+  }
+  uint64_t c = 0;
+  int x = S.g0[0], y = S.g0[1], z = S.g0[2];
+  bool row_ok = false;
+  RowTerms R;
+  while (__any(c < count)) {
+    bool hit = false;
+    K key = 0;
+    if (c < count) {
+      if (x == S.g0[0]) row_ok = row_terms(S, y, z, R);
+      if (row_ok) {
+        hit = cell_hit(S, R, x);
+        key = mesh_key | morton3<K>((uint32_t)x, (uint32_t)y, (uint32_t)z);
+      }
+      if (row_ok && x < S.g1[0]) {
+        ++x;
+        ++c;
+      } else {  // end of row, or an empty row skipped as a whole
+        c += (uint64_t)(S.g1[0] - x + 1);
+        x = S.g0[0];
+        if (++y > S.g1[1]) {
+          y = S.g0[1];
+          ++z;
+        }
+      }
+    }
+    st.push(hit, key);
+  }
+  st.finish();
+}
+
+template <typename K>
+__global__ __launch_bounds__(BLOCK) void k_vox_big(const MeshDev* meshes,
+                                                   const MeshParams* params,
+                                                   const uint64_t* tri_base, uint32_t n_meshes,
+                                                   int cb, uint32_t chunk_cells,
+                                                   const BigItem* big, uint64_t big_cap,
+                                                   const unsigned long long* big_count,
+                                                   uint64_t n_tris, K* keys,
+                                                   unsigned long long* key_count) {
+  __shared__ K stage[WAVES][STAGE];
+  Stager<K> st;
+  st.buf = stage[threadIdx.x >> 6];
+  st.keys = keys;
+  st.count = key_count;
+  const uint64_t item_id = (uint64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  const uint64_t queued = *big_count;
+  const uint64_t n_items = queued < big_cap ? queued : big_cap;
+  if (item_id < n_items && big[item_id].tri < n_tris) {
+    const BigItem it = big[item_id];
+    const uint32_t m = find_mesh(tri_base, n_meshes, it.tri);
+    const MeshDev& md = meshes[m];
+    const MeshParams P = params[m];
+    float v[3][3];
+    load_tri(md, P, it.tri - md.tri_base, v);  // indices were checked when queued
+    TriSetup S;
+    tri_setup(v, P, S);
+    const int nx = S.g1[0] - S.g0[0] + 1, ny = S.g1[1] - S.g0[1] + 1,
+              nz = S.g1[2] - S.g0[2] + 1;
+    const uint64_t rows = (uint64_t)ny * nz;
+    const uint64_t rpc = rows_per_chunk(chunk_cells, nx);
+    const uint64_t r_end = it.row0 + rpc < rows ? it.row0 + rpc : rows;
+    const K mesh_key = (K)m << cb;
+    for (uint64_t r = it.row0; r < r_end; ++r) {
+      const int y = S.g0[1] + (int)(r % (uint64_t)ny);
+      const int z = S.g0[2] + (int)(r / (uint64_t)ny);
+      RowTerms R;
+      if (!row_terms(S, y, z, R)) continue;
+      for (int xb = S.g0[0]; xb <= S.g1[0]; xb += 64) {
+        const int x = xb + (int)__lane_id();
+        const bool hit = x <= S.g1[0] && cell_hit(S, R, x);
+        st.push(hit, mesh_key | morton3<K>((uint32_t)x, (uint32_t)y, (uint32_t)z));
+      }
+    }
+  }
+  st.finish();
+}
+
+}  // namespace
+
+void launch_bbox(const MeshDev* meshes, uint32_t n_meshes, uint64_t max_points, uint32_t* acc,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_bbox_init, dim3((n_meshes * 8 + 255) / 256), dim3(256), 0, s, acc,
+                     n_meshes);
+  if (max_points == 0) return;
+  const uint32_t gx = (uint32_t)((max_points + BBOX_CHUNK - 1) / BBOX_CHUNK);
+  hipLaunchKernelGGL(k_bbox, dim3(gx, n_meshes), dim3(BLOCK), 0, s, meshes, acc);
+}
+
+void launch_params(const MeshDev* meshes, uint32_t n_meshes, const uint32_t* acc,
+                   MeshParams* params, hipStream_t s) {
+  hipLaunchKernelGGL(k_params, dim3((n_meshes + 63) / 64), dim3(64), 0, s, meshes, n_meshes, acc,
+                     params);
+}
+
+void launch_classify(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
+                     uint32_t n_meshes, uint64_t n_tris, const Tuning& tn,
+                     unsigned long long* totals, hipStream_t s) {
+  if (n_tris == 0) return;
+  const uint32_t blocks = (uint32_t)((n_tris + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_flag_bad_index, dim3(blocks), dim3(BLOCK), 0, s, meshes,
+                     const_cast<MeshParams*>(params), tri_base, n_meshes, n_tris);
+  hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(BLOCK), 0, s, meshes, params, tri_base,
+                     n_meshes, n_tris, tn.small_max_cells, tn.chunk_cells, totals);
+}
+
+template <typename K>
+void launch_voxelize(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
+                     uint32_t n_meshes, uint64_t n_tris, int cb, const Tuning& tn, K* keys,
+                     unsigned long long* key_count, BigItem* big, unsigned long long* big_count,
+                     uint64_t big_cap, hipStream_t s) {
+  if (n_tris == 0) return;
+  const uint32_t blocks = (uint32_t)((n_tris + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_vox_small<K>, dim3(blocks), dim3(BLOCK), 0, s, meshes, params, tri_base,
+                     n_meshes, n_tris, cb, tn.small_max_cells, tn.chunk_cells, keys, key_count,
+                     big, big_count, big_cap);
+  if (big_cap == 0) return;
+  const uint32_t bblocks = (uint32_t)((big_cap + WAVES - 1) / WAVES);
+  hipLaunchKernelGGL(k_vox_big<K>, dim3(bblocks), dim3(BLOCK), 0, s, meshes, params, tri_base,
+                     n_meshes, cb, tn.chunk_cells, (const BigItem*)big, big_cap,
+                     (const unsigned long long*)big_count, n_tris, keys, key_count);
+}
+
+template void launch_voxelize<uint32_t>(const MeshDev*, const MeshParams*, const uint64_t*,
+                                        uint32_t, uint64_t, int, const Tuning&, uint32_t*,
+                                        unsigned long long*, BigItem*, unsigned long long*,
+                                        uint64_t, hipStream_t);
+template void launch_voxelize<uint64_t>(const MeshDev*, const MeshParams*, const uint64_t*,
+                                        uint32_t, uint64_t, int, const Tuning&, uint64_t*,
+                                        unsigned long long*, BigItem*, unsigned long long*,
+                                        uint64_t, hipStream_t);
+
+}  // namespace svo

@@ -1,0 +1,163 @@
+// kernels_api.cpp -- include/svo/svo_kernels.h: single kernels on host buffers.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "svo/svo_kernels.h"
+#include "svo_host.h"
+
+using namespace svo;
+
+namespace {
+
+struct DevBufs {
+  std::vector<void*> ptrs;
+  ~DevBufs() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  void* get(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 256) != hipSuccess) return nullptr;
+    ptrs.push_back(p);
+    return p;
+  }
+};
+
+uint32_t blocks_for(uint64_t n, uint64_t per, uint32_t max_blocks) {
+  const uint64_t b = (n + per - 1) / per;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, max_blocks));
+}
+
+}  // namespace
+
+extern "C" {
+
+int svo_k_radix_sort(int device, void* keys, uint64_t n, int key_bytes, int bits) {
+  if ((key_bytes != 4 && key_bytes != 8) || bits < 0 || bits > 8 * key_bytes ||
+      (n && !keys) || n >= (1ull << 32))
+    return SVO_E_INVALID;
+  if (n == 0) return SVO_OK;
+  if (hipSetDevice(device) != hipSuccess) return SVO_E_HIP;
+  DevBufs B;
+  const uint32_t blocks = blocks_for(n, 16384, 1024);
+  void* a = B.get(key_bytes * n);
+  void* b = B.get(key_bytes * n);
+  void* tmp = B.get(radix_sort_tmp_bytes(blocks));
+  unsigned long long* cnt = (unsigned long long*)B.get(8);
+  if (!a || !b || !tmp || !cnt) return SVO_E_OOM;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return SVO_E_HIP;
+  int st = SVO_OK;
+  const unsigned long long nn = n;
+  if (hipMemcpyAsync(a, keys, key_bytes * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(cnt, &nn, 8, hipMemcpyHostToDevice, s) != hipSuccess) {
+    st = SVO_E_HIP;
+  } else {
+    void* out = nullptr;
+    if (key_bytes == 4) {
+      uint32_t* r = nullptr;
+      launch_radix_sort<uint32_t>((uint32_t*)a, (uint32_t*)b, cnt, n, bits, blocks, tmp, &r, s);
+      out = r;
+    } else {
+      uint64_t* r = nullptr;
+      launch_radix_sort<uint64_t>((uint64_t*)a, (uint64_t*)b, cnt, n, bits, blocks, tmp, &r, s);
+      out = r;
+    }
+    if (hipMemcpyAsync(keys, out, key_bytes * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+      st = SVO_E_HIP;
+  }
+  (void)hipStreamDestroy(s);
+  return st;
+}
+
+int svo_k_voxelize(int device, const float* xyz, uint64_t n_points, const int32_t* tri,
+                   uint64_t n_tris, const int32_t* res, uint32_t key_bits, void* codes,
+                   uint64_t cap, uint64_t* n_hits) {
+  svo_mesh_desc d{xyz, n_points, tri, n_tris};
+  svo_params p{};
+  for (int a = 0; a < 3; ++a) p.res[a] = res ? res[a] : 0;
+  p.key_bits = key_bits;
+  int kb = 32;
+  int st = validate_params(&d, &p, &kb);
+  if (st != SVO_OK) return st;
+  if (hipSetDevice(device) != hipSuccess) return SVO_E_HIP;
+  int bits = 1;
+  for (int a = 0; a < 3; ++a)
+    while ((1ll << bits) < (long long)res[a]) ++bits;
+  DevBufs B;
+  float* dp = (float*)B.get(12 * n_points);
+  int32_t* dt = (int32_t*)B.get(12 * n_tris);
+  MeshDev* md = (MeshDev*)B.get(sizeof(MeshDev));
+  uint64_t* tb = (uint64_t*)B.get(16);
+  uint32_t* acc = (uint32_t*)B.get(32);
+  MeshParams* par = (MeshParams*)B.get(sizeof(MeshParams));
+  unsigned long long* cnt = (unsigned long long*)B.get(64);
+  if (!dp || !dt || !md || !tb || !acc || !par || !cnt) return SVO_E_OOM;
+  MeshDev h{};
+  h.pts = dp;
+  h.tri = dt;
+  h.n_points = n_points;
+  h.n_tris = n_tris;
+  for (int a = 0; a < 3; ++a) h.res[a] = res[a];
+  const uint64_t htb[2] = {0, n_tris};
+  const unsigned long long zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return SVO_E_HIP;
+  Tuning tn;
+  st = SVO_E_HIP;
+  unsigned long long tot[4] = {0, 0, 0, 0};
+  if (hipMemcpyAsync(dp, xyz, 12 * n_points, hipMemcpyHostToDevice, s) == hipSuccess &&
+      hipMemcpyAsync(dt, tri, 12 * n_tris, hipMemcpyHostToDevice, s) == hipSuccess &&
+      hipMemcpyAsync(md, &h, sizeof h, hipMemcpyHostToDevice, s) == hipSuccess &&
+      hipMemcpyAsync(tb, htb, 16, hipMemcpyHostToDevice, s) == hipSuccess &&
+      hipMemcpyAsync(cnt, zero, 64, hipMemcpyHostToDevice, s) == hipSuccess) {
+    launch_bbox(md, 1, n_points, acc, s);
+    launch_params(md, 1, acc, par, s);
+    launch_classify(md, par, tb, 1, n_tris, tn, cnt, s);
+    if (hipMemcpyAsync(tot, cnt, 32, hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess)
+      st = SVO_OK;
+  }
+  MeshParams hp{};
+  if (st == SVO_OK) {
+    st = SVO_E_HIP;
+    if (hipMemcpyAsync(&hp, par, sizeof hp, hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess)
+      st = hp.status ? SVO_E_INVALID : SVO_OK;
+  }
+  if (st == SVO_OK) {
+    const uint64_t cells = std::max<unsigned long long>(tot[0], 1), n_big = tot[1];
+    const size_t kbytes = kb == 64 ? 8 : 4;
+    void* keys = B.get(kbytes * cells);
+    BigItem* big = (BigItem*)B.get(sizeof(BigItem) * std::max<uint64_t>(n_big, 1));
+    if (!keys || !big) {
+      st = SVO_E_OOM;
+    } else {
+      if (kb == 64)
+        launch_voxelize<uint64_t>(md, par, tb, 1, n_tris, 3 * bits, tn, (uint64_t*)keys, cnt + 2,
+                                  big, cnt + 3, n_big, s);
+      else
+        launch_voxelize<uint32_t>(md, par, tb, 1, n_tris, 3 * bits, tn, (uint32_t*)keys, cnt + 2,
+                                  big, cnt + 3, n_big, s);
+      st = SVO_E_HIP;
+      if (hipMemcpyAsync(tot, cnt, 32, hipMemcpyDeviceToHost, s) == hipSuccess &&
+          hipStreamSynchronize(s) == hipSuccess) {
+        const uint64_t h_hits = tot[2];
+        if (n_hits) *n_hits = h_hits;
+        const uint64_t k = std::min<uint64_t>(h_hits, cap);
+        if (k && codes) {
+          if (hipMemcpyAsync(codes, keys, kbytes * k, hipMemcpyDeviceToHost, s) == hipSuccess &&
+              hipStreamSynchronize(s) == hipSuccess)
+            st = SVO_OK;
+        } else {
+          st = SVO_OK;
+        }
+      }
+    }
+  }
+  (void)hipStreamDestroy(s);
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,685 @@
+// pipeline.cpp -- one pass of the SVO build over a batch of meshes.
+//
+// Host C++ drives the gfx950 kernels on one stream (a "lane"):
+//
+//   A0  bbox -> grid params -> triangle classification     (1 readback: box-cell
+//                                                            total, big chunks)
+//   A1  voxelize -> radix sort -> unique + level 1 -> levels 2..L
+//                                                         (1 readback: per-mesh
+//                                                          depth / level spans)
+//   B   node records -> roots -> voxels -> neighbors -> vertex/edge/face
+//                                                         (1 readback: totals)
+//
+// The readbacks size the next phase's buffers exactly; inside a phase every
+// launch shape is fixed by host-known bounds and every data-dependent count
+// is read by the kernels from device memory.  A single mesh is a batch of one;
+// a batch sorts composite (mesh, Morton) keys, so all meshes share every
+// launch.  Reference correspondence: SparseVoxelOctree::create (SVO.cu:341-520)
+// and constructNodeAtrributes (SVO.cu:730-735).
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "svo_host.h"
+
+namespace svo {
+
+#define HIP_TRY(expr)                  \
+  do {                                 \
+    hipError_t e_ = (expr);            \
+    if (e_ != hipSuccess) return SVO_E_HIP; \
+  } while (0)
+#define SVO_TRY(expr)       \
+  do {                      \
+    int s_ = (expr);        \
+    if (s_ != SVO_OK) return s_; \
+  } while (0)
+
+static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// ------------------------------------------------------------------ context
+int Context::init(int dev, uint64_t bud) {
+  device = dev;
+  budget = bud;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || dev < 0 || dev >= n) return SVO_E_HIP;
+  HIP_TRY(hipSetDevice(dev));
+  HIP_TRY(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
+  return SVO_OK;
+}
+
+void Context::destroy() {
+  hipSetDevice(device);
+  for (auto& l : lanes) {
+    if (l->stream) hipStreamSynchronize(l->stream);
+    l->ws.release_all();
+    for (auto e : l->events) hipEventDestroy(e);
+    if (l->pinned) hipHostFree(l->pinned);
+    if (l->stream) hipStreamDestroy(l->stream);
+  }
+  lanes.clear();
+  free_lanes.clear();
+  if (aux_stream) hipStreamDestroy(aux_stream);
+  aux_stream = nullptr;
+}
+
+Lane* Context::acquire_lane() {
+  std::unique_lock<std::mutex> lk(mu);
+  if (free_lanes.empty() && lanes.size() < 8) {
+    auto l = std::make_unique<Lane>();
+    hipSetDevice(device);
+    if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    l->ws.ctx = this;
+    free_lanes.push_back(l.get());
+    lanes.push_back(std::move(l));
+  }
+  cv.wait(lk, [&] { return !free_lanes.empty(); });
+  Lane* l = free_lanes.back();
+  free_lanes.pop_back();
+  return l;
+}
+
+void Context::release_lane(Lane* l) {
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    free_lanes.push_back(l);
+  }
+  cv.notify_one();
+}
+
+int Context::reserve(uint64_t bytes) {
+  uint64_t cur = in_use.load();
+  while (true) {
+    if (budget && cur + bytes > budget) return SVO_E_OOM;
+    if (in_use.compare_exchange_weak(cur, cur + bytes)) return SVO_OK;
+  }
+}
+void Context::release(uint64_t bytes) { in_use.fetch_sub(bytes); }
+
+int Context::dev_malloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  SVO_TRY(reserve(bytes));
+  hipSetDevice(device);
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    hipGetLastError();
+    release(bytes);
+    *p = nullptr;
+    return SVO_E_OOM;
+  }
+  return SVO_OK;
+}
+
+void Context::dev_free(void* p, size_t bytes) {
+  if (!p) return;
+  hipSetDevice(device);
+  hipFree(p);
+  release(bytes ? bytes : 256);
+}
+
+DeviceBlock::~DeviceBlock() {
+  if (ctx && ptr) ctx->dev_free(ptr, bytes);
+}
+
+int Workspace::get(const char* name, size_t bytes, void** out) {
+  bytes = align256(bytes ? bytes : 256);
+  Buf& b = bufs[name];
+  if (b.bytes < bytes) {
+    if (b.ptr) ctx->dev_free(b.ptr, b.bytes);
+    b.ptr = nullptr;
+    b.bytes = 0;
+    // grow with headroom so that slowly growing inputs do not reallocate
+    const size_t want = bytes + bytes / 8;
+    void* p = nullptr;
+    int st = ctx->dev_malloc(&p, want);
+    if (st != SVO_OK) {
+      st = ctx->dev_malloc(&p, bytes);
+      if (st != SVO_OK) return st;
+      b.bytes = bytes;
+    } else {
+      b.bytes = want;
+    }
+    b.ptr = p;
+  }
+  *out = b.ptr;
+  return SVO_OK;
+}
+
+void Workspace::release_all() {
+  for (auto& kv : bufs)
+    if (kv.second.ptr) ctx->dev_free(kv.second.ptr, kv.second.bytes);
+  bufs.clear();
+}
+
+size_t Workspace::total() const {
+  size_t s = 0;
+  for (auto& kv : bufs) s += kv.second.bytes;
+  return s;
+}
+
+int Lane::ensure_pinned(size_t bytes) {
+  if (pinned_bytes >= bytes) return SVO_OK;
+  if (pinned) hipHostFree(pinned);
+  pinned = nullptr;
+  pinned_bytes = 0;
+  const size_t want = std::max<size_t>(bytes, 1 << 16);
+  if (hipHostMalloc(&pinned, want, hipHostMallocDefault) != hipSuccess) return SVO_E_HIP;
+  pinned_bytes = want;
+  return SVO_OK;
+}
+
+// ------------------------------------------------------------------ validation
+static int bits_for(int32_t r) {
+  int b = 0;
+  while ((1ll << b) < (long long)r) ++b;
+  return b;
+}
+
+int validate_params(const svo_mesh_desc* mesh, const svo_params* p, int* key_bits_out) {
+  if (!mesh || !p) return SVO_E_INVALID;
+  if (mesh->n_points == 0 || mesh->n_tris == 0) return SVO_E_EMPTY;
+  if (!mesh->xyz || !mesh->tri) return SVO_E_INVALID;
+  if (p->semantics > SVO_SEM_INTENDED) return SVO_E_INVALID;
+  if (mesh->n_points >= (1ull << 31) || mesh->n_tris >= (1ull << 32)) return SVO_E_INVALID;
+  int kb = (int)p->key_bits;
+  int32_t rmax = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (p->res[a] < 1) return SVO_E_INVALID;
+    rmax = std::max(rmax, p->res[a]);
+  }
+  if (kb == 0) kb = rmax <= 1024 ? 32 : 64;
+  if (kb != 32 && kb != 64) return SVO_E_INVALID;
+  // uint32 codes: bit 31 is the reference's flag and the 16-bit-coordinate
+  // encoder drops bit 32 (MortonLUT.h:482-501, SVO.cu:180-182): res <= 1024.
+  if (kb == 32 && rmax > 1024) return SVO_E_RES_LIMIT;
+  // uint64 codes: coordinates stay within the reference's uint16 loop range.
+  if (kb == 64 && rmax > 65536) return SVO_E_RES_LIMIT;
+  if (key_bits_out) *key_bits_out = kb;
+  return SVO_OK;
+}
+
+// Host-side device-memory estimate for a build: surface voxels ~ 4.7 res^2
+// for a closed surface spanning the box, nodes ~ 2.7 per voxel, output
+// ~630 B per node (records of the reference layout), plus key workspace.
+uint64_t estimate_bytes(uint64_t n_points, uint64_t n_tris, const svo_params* p) {
+  const double r = std::max({p->res[0], p->res[1], p->res[2]});
+  const double voxels = std::min(4.7 * r * r * 2.0, r * r * r);
+  const double hits = std::max(voxels * 3.0, (double)n_tris * 4.0);
+  const double nodes = 2.7 * voxels;
+  const double per_node = (p->flags & SVO_BUILD_NO_VERTEX_EDGE) ? 170.0
+                          : (p->semantics == SVO_SEM_INTENDED ? 800.0 : 640.0);
+  const double b = nodes * per_node + hits * 8.0 * 2.5 + (double)n_points * 12.0 +
+                   (double)n_tris * 12.0 + (1 << 20);
+  return (uint64_t)b;
+}
+
+// ------------------------------------------------------------------ timing
+struct StageTimer {
+  Context& C;
+  Lane& L;
+  std::vector<std::string> names;
+  size_t used = 0;
+  StageTimer(Context& c, Lane& l) : C(c), L(l) {}
+  void mark(const char* name) {
+    if (!C.timing) return;
+    if (L.events.size() <= used) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return;
+      L.events.push_back(e);
+    }
+    hipEventRecord(L.events[used], L.stream);
+    names.push_back(name);
+    ++used;
+  }
+  void collect() {
+    if (!C.timing || used < 2) return;
+    std::vector<double> ms;
+    std::vector<std::string> nm;
+    for (size_t i = 1; i < used; ++i) {
+      float t = 0;
+      hipEventElapsedTime(&t, L.events[i - 1], L.events[i]);
+      ms.push_back(t);
+      nm.push_back(names[i]);
+    }
+    std::lock_guard<std::mutex> lk(C.stage_mu);
+    C.stage_ms = ms;
+    C.stage_names = nm;
+  }
+};
+
+// ------------------------------------------------------------------ the pass
+namespace {
+
+struct Plan {
+  std::vector<uint32_t> idx;  // active mesh -> input index
+  int bits = 1;               // per-axis coordinate bits of the batch
+  int cb = 3;                 // code bits
+  int mb = 0;                 // mesh bits
+  bool k64 = false;
+  int n_levels = 2;
+  int code_bytes = 4;
+};
+
+template <typename K>
+int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan& plan,
+              std::vector<svo_tree*>& trees, std::vector<int>& statuses) {
+  hipStream_t s = L.stream;
+  Workspace& W = L.ws;
+  const uint32_t n = (uint32_t)plan.idx.size();
+  const uint32_t NL = (uint32_t)plan.n_levels;
+  StageTimer timer(C, L);
+  KeyLayout<K> kl;
+  kl.cb = plan.cb;
+  kl.cmask = (plan.cb >= (int)(8 * sizeof(K))) ? (K)~(K)0 : (K)(((K)1 << plan.cb) - 1);
+
+  // ---- inputs on the device
+  std::vector<MeshDev> md(n);
+  std::vector<uint64_t> tri_base(n + 1, 0);
+  uint64_t host_bytes = 0, max_points = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const MeshInput& mi = in[plan.idx[i]];
+    if (!mi.on_device) host_bytes += align256(12 * mi.desc.n_points) + align256(12 * mi.desc.n_tris);
+  }
+  char* staging = nullptr;
+  if (host_bytes) SVO_TRY(W.get("input", host_bytes, (void**)&staging));
+  uint64_t soff = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    const MeshInput& mi = in[plan.idx[i]];
+    MeshDev& d = md[i];
+    d.n_points = mi.desc.n_points;
+    d.n_tris = mi.desc.n_tris;
+    if (mi.on_device) {
+      d.pts = mi.desc.xyz;
+      d.tri = mi.desc.tri;
+    } else {
+      float* dp = (float*)(staging + soff);
+      soff += align256(12 * mi.desc.n_points);
+      int32_t* dt = (int32_t*)(staging + soff);
+      soff += align256(12 * mi.desc.n_tris);
+      HIP_TRY(hipMemcpyAsync(dp, mi.desc.xyz, 12 * mi.desc.n_points, hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemcpyAsync(dt, mi.desc.tri, 12 * mi.desc.n_tris, hipMemcpyHostToDevice, s));
+      d.pts = dp;
+      d.tri = dt;
+    }
+    for (int a = 0; a < 3; ++a) d.res[a] = mi.params.res[a];
+    d.tri_base = tri_base[i];
+    d.pt_base = 0;
+    tri_base[i + 1] = tri_base[i] + d.n_tris;
+    max_points = std::max<uint64_t>(max_points, d.n_points);
+  }
+  const uint64_t T = tri_base[n];
+
+  // small control buffers: mesh descriptors, tri bases, bbox, params, counters
+  const size_t sz_md = align256(sizeof(MeshDev) * n), sz_tb = align256(8 * (n + 1)),
+               sz_acc = align256(32 * n), sz_par = align256(sizeof(MeshParams) * n),
+               sz_cnt = align256(8 * 8);
+  char* ctl = nullptr;
+  SVO_TRY(W.get("ctl", sz_md + sz_tb + sz_acc + sz_par + sz_cnt, (void**)&ctl));
+  MeshDev* d_md = (MeshDev*)ctl;
+  uint64_t* d_tb = (uint64_t*)(ctl + sz_md);
+  uint32_t* d_acc = (uint32_t*)(ctl + sz_md + sz_tb);
+  MeshParams* d_par = (MeshParams*)(ctl + sz_md + sz_tb + sz_acc);
+  unsigned long long* d_cnt = (unsigned long long*)(ctl + sz_md + sz_tb + sz_acc + sz_par);
+  // d_cnt: [0] box cells, [1] big chunks, [2] keys emitted, [3] big queued
+  const size_t pin_small = 64 + sizeof(MeshParams) * n;
+  SVO_TRY(L.ensure_pinned(pin_small));
+  {
+    // stage host->device control data through the pinned buffer
+    std::memcpy(L.pinned, md.data(), sizeof(MeshDev) * n);
+    HIP_TRY(hipMemcpyAsync(d_md, L.pinned, sizeof(MeshDev) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::memcpy(L.pinned, tri_base.data(), 8 * (n + 1));
+    HIP_TRY(hipMemcpyAsync(d_tb, L.pinned, 8 * (n + 1), hipMemcpyHostToDevice, s));
+  }
+  launch_fill_u32((uint32_t*)d_cnt, 0u, 16, s);
+  timer.mark("start");
+
+  // ---- A0: bbox, params, classification
+  launch_bbox(d_md, n, max_points, d_acc, s);
+  launch_params(d_md, n, d_acc, d_par, s);
+  launch_classify(d_md, d_par, d_tb, n, T, C.tuning, d_cnt, s);
+  timer.mark("classify");
+  HIP_TRY(hipMemcpyAsync(L.pinned, d_cnt, 16, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  uint64_t cells = ((uint64_t*)L.pinned)[0];
+  const uint64_t n_big = ((uint64_t*)L.pinned)[1];
+  if (cells >= (1ull << 32) - 1) return SVO_E_OOM;  // key counts are 32-bit in the sort
+  const uint64_t key_cap = std::max<uint64_t>(cells, 1);
+
+  // ---- workspace for A1
+  K *keys_a = nullptr, *keys_b = nullptr;
+  BigItem* big = nullptr;
+  void *sort_tmp = nullptr, *lvl_tmp = nullptr;
+  SVO_TRY(W.get("keys_a", sizeof(K) * key_cap, (void**)&keys_a));
+  SVO_TRY(W.get("keys_b", sizeof(K) * key_cap, (void**)&keys_b));
+  SVO_TRY(W.get("big", sizeof(BigItem) * std::max<uint64_t>(n_big, 1), (void**)&big));
+  // fixed grids sized from host-known capacities (small inputs get small grids)
+  auto grid_for = [](uint64_t cap_elems, uint64_t per_block, uint32_t max_blocks) {
+    const uint64_t b = (cap_elems + per_block - 1) / per_block;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, max_blocks));
+  };
+  const uint32_t sort_blocks = grid_for(key_cap, 16384, C.tuning.sort_blocks);
+  SVO_TRY(W.get("sort_tmp", radix_sort_tmp_bytes(C.tuning.sort_blocks), &sort_tmp));
+  SVO_TRY(W.get("lvl_tmp", level_tmp_bytes(C.tuning.scan_blocks), &lvl_tmp));
+  // level capacities: cap_1 = box cells (>= hits >= unique voxels);
+  // cap_l = min(cap_{l-1}, number of level-l cells over the batch)
+  std::vector<uint64_t> cap(NL + 2, 0);
+  cap[1] = key_cap;
+  for (uint32_t l = 2; l <= NL + 1; ++l) {
+    uint64_t cellsl = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      uint64_t c = 1;
+      for (int a = 0; a < 3; ++a) {
+        const uint64_t r = (uint64_t)md[i].res[a];
+        const uint64_t sh = l - 1;
+        c *= sh >= 63 ? 1 : std::max<uint64_t>(1, (r + (1ull << sh) - 1) >> sh);
+      }
+      cellsl += c;
+    }
+    cap[l] = std::max<uint64_t>(1, std::min(cap[l - 1], cellsl));
+  }
+  std::vector<LevelArrays> lv(NL + 2);
+  {
+    size_t total = 0;
+    for (uint32_t l = 1; l <= NL + 1; ++l)
+      total += align256(sizeof(K) * cap[l]) + align256(4 * cap[l]) + (l >= 2 ? align256(4 * cap[l]) : 0);
+    char* base = nullptr;
+    SVO_TRY(W.get("levels", total, (void**)&base));
+    for (uint32_t l = 1; l <= NL + 1; ++l) {
+      lv[l].cap = cap[l];
+      lv[l].elem = base;
+      base += align256(sizeof(K) * cap[l]);
+      lv[l].elem_group = (uint32_t*)base;
+      base += align256(4 * cap[l]);
+      if (l >= 2) {
+        lv[l].first_child = (uint32_t*)base;
+        base += align256(4 * cap[l]);
+      } else {
+        lv[l].first_child = nullptr;
+      }
+    }
+  }
+  const size_t sz_counts = align256(8 * (NL + 2)), sz_spans = align256(sizeof(LevelSpan) * n * NL),
+               sz_state = align256(sizeof(MeshTreeState) * n);
+  char* lctl = nullptr;
+  SVO_TRY(W.get("lctl", sz_counts + sz_spans + sz_state, (void**)&lctl));
+  unsigned long long* d_counts = (unsigned long long*)lctl;
+  LevelSpan* d_spans = (LevelSpan*)(lctl + sz_counts);
+  MeshTreeState* d_state = (MeshTreeState*)(lctl + sz_counts + sz_spans);
+  launch_fill_u32((uint32_t*)lctl, 0u, (sz_counts + sz_spans + sz_state) / 4, s);
+
+  // ---- A1: voxelize, sort, levels
+  timer.mark("setup_a1");
+  launch_voxelize<K>(d_md, d_par, d_tb, n, T, plan.cb, C.tuning, keys_a, d_cnt + 2, big,
+                     d_cnt + 3, n_big, s);
+  timer.mark("voxelize");
+  K* sorted = nullptr;
+  launch_radix_sort<K>(keys_a, keys_b, d_cnt + 2, key_cap, plan.cb + plan.mb, sort_blocks,
+                       sort_tmp, &sorted, s);
+  timer.mark("sort");
+  launch_level_first<K>(sorted, d_cnt + 2, kl, lv[1], lv[2], d_counts, d_spans, NL,
+                        grid_for(key_cap, 8192, C.tuning.scan_blocks), lvl_tmp, s);
+  launch_level_first_finalize(d_spans, n, NL, d_state, s);
+  timer.mark("unique_level1");
+  for (uint32_t l = 2; l <= NL; ++l)
+    launch_level<K>((int)l, kl, lv[l], lv[l + 1], d_counts, d_spans, n, NL, d_state,
+                    grid_for(cap[l], 8192, C.tuning.scan_blocks), lvl_tmp, s);
+  timer.mark("levels");
+
+  // ---- readback of the level phase
+  const size_t rb = 8 * (NL + 2) + sizeof(LevelSpan) * n * NL + sizeof(MeshTreeState) * n +
+                    sizeof(MeshParams) * n + 64;
+  SVO_TRY(L.ensure_pinned(rb));
+  char* hp = (char*)L.pinned;
+  uint64_t* h_counts = (uint64_t*)hp;
+  LevelSpan* h_spans = (LevelSpan*)(hp + 8 * (NL + 2));
+  MeshTreeState* h_state = (MeshTreeState*)((char*)h_spans + sizeof(LevelSpan) * n * NL);
+  MeshParams* h_par = (MeshParams*)((char*)h_state + sizeof(MeshTreeState) * n);
+  uint64_t* h_hits = (uint64_t*)((char*)h_par + sizeof(MeshParams) * n);
+  HIP_TRY(hipMemcpyAsync(h_counts, d_counts, 8 * (NL + 2), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h_spans, d_spans, sizeof(LevelSpan) * n * NL, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h_state, d_state, sizeof(MeshTreeState) * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h_par, d_par, sizeof(MeshParams) * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h_hits, d_cnt + 2, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const uint64_t n_hits = *h_hits;
+  if (n_hits > key_cap) return SVO_E_HIP;  // cannot happen: hits <= box cells
+  // the pinned area is reused below: keep host copies of the readback
+  const std::vector<uint64_t> v_counts(h_counts, h_counts + NL + 2);
+  const std::vector<LevelSpan> v_spans(h_spans, h_spans + (size_t)n * NL);
+  const std::vector<MeshTreeState> v_state(h_state, h_state + n);
+  const std::vector<MeshParams> v_par(h_par, h_par + n);
+  h_counts = const_cast<uint64_t*>(v_counts.data());
+  h_spans = const_cast<LevelSpan*>(v_spans.data());
+  h_state = const_cast<MeshTreeState*>(v_state.data());
+  h_par = const_cast<MeshParams*>(v_par.data());
+
+  // ---- per-mesh node layout
+  std::vector<MeshNodeLayout> lay(n);
+  std::vector<uint64_t> node_base(n + 1, 0);
+  std::vector<int> mstat(n, SVO_OK);
+  uint64_t N = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    MeshNodeLayout& Lm = lay[i];
+    std::memset(&Lm, 0, sizeof(Lm));
+    const MeshParams& P = h_par[i];
+    const uint32_t d = h_state[i].depth;
+    if (P.status != 0) mstat[i] = SVO_E_INVALID;
+    else if (d == 0) mstat[i] = SVO_E_EMPTY;
+    Lm.node_base = N;
+    if (mstat[i] == SVO_OK) {
+      Lm.depth = d;
+      uint64_t run = 0;
+      for (uint32_t l = 1; l < d; ++l) {
+        const LevelSpan& sp = h_spans[(uint64_t)i * NL + (l - 1)];
+        Lm.level_off[l - 1] = run;
+        run += 8ull * (sp.group_end - sp.group_first);
+      }
+      Lm.level_off[d - 1] = run;
+      Lm.n_nodes = run + 1;
+      float w = P.unit[0];  // unitNodeWidth = unitVoxelSize.x(), doubled per level (SVO.cu:483)
+      for (uint32_t l = 0; l < 24; ++l) {
+        Lm.width[l] = w;
+        w = (float)((double)w * 2.0);
+      }
+      if (Lm.n_nodes >= (1ull << 32)) mstat[i] = SVO_E_RES_LIMIT;  // uint32 node links
+    }
+    if (mstat[i] != SVO_OK) Lm.n_nodes = 0;
+    Lm.key_out_bits = plan.code_bytes * 8;
+    N += Lm.n_nodes;
+    node_base[i + 1] = N;
+  }
+  const uint64_t U1 = h_counts[0];
+  const int sem = (int)in[plan.idx[0]].params.semantics;
+  const bool want_attrs = !(in[plan.idx[0]].params.flags & SVO_BUILD_NO_VERTEX_EDGE);
+
+  // ---- output block
+  const int cbytes = plan.code_bytes;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += align256(bytes);
+    return o;
+  };
+  const size_t o_code = take(cbytes * N), o_leaf = take(N), o_org = take(12 * N),
+               o_w = take(4 * N), o_par = take(4 * N), o_ch = take(32 * N),
+               o_nb = take(108 * N), o_vx = take(cbytes * U1);
+  const size_t o_v = take(want_attrs ? 16 * 8 * N : 0);
+  const size_t o_e = take(want_attrs ? 28 * 12 * N : 0);
+  const size_t o_f = take(want_attrs && sem == SVO_SEM_INTENDED ? 28 * 6 * N : 0);
+  auto out = std::make_shared<BatchOutput>();
+  out->block = std::make_shared<DeviceBlock>();
+  {
+    void* p = nullptr;
+    SVO_TRY(C.dev_malloc(&p, off));
+    out->block->ctx = &C;
+    out->block->ptr = p;
+    out->block->bytes = off;
+  }
+  char* ob = (char*)out->block->ptr;
+  NodeArrays& na = out->nodes;
+  na.code = ob + o_code;
+  na.is_leaf = (uint8_t*)(ob + o_leaf);
+  na.origin = (float*)(ob + o_org);
+  na.width = (float*)(ob + o_w);
+  na.parent = (uint32_t*)(ob + o_par);
+  na.childs = (uint32_t*)(ob + o_ch);
+  na.neighbors = (uint32_t*)(ob + o_nb);
+  na.code_bytes = cbytes;
+  out->voxels = ob + o_vx;
+  out->verts = (svo_vertex*)(ob + o_v);
+  out->edges = (svo_edge*)(ob + o_e);
+  out->faces = (svo_edge*)(ob + o_f);
+  out->code_bytes = cbytes;
+
+  // ---- B: node records, neighbors, vertex/edge/face records
+  char* bctl = nullptr;
+  const size_t sz_lay = align256(sizeof(MeshNodeLayout) * n), sz_nb = align256(8 * (n + 1)),
+               sz_tot = align256(8 * (4 + 3 * n));
+  SVO_TRY(W.get("bctl", sz_lay + sz_nb + sz_tot, (void**)&bctl));
+  MeshNodeLayout* d_lay = (MeshNodeLayout*)bctl;
+  uint64_t* d_nbase = (uint64_t*)(bctl + sz_lay);
+  unsigned long long* d_tot = (unsigned long long*)(bctl + sz_lay + sz_nb);
+  SVO_TRY(L.ensure_pinned(sz_lay + sz_nb + sz_tot));
+  std::memcpy(L.pinned, lay.data(), sizeof(MeshNodeLayout) * n);
+  std::memcpy((char*)L.pinned + sz_lay, node_base.data(), 8 * (n + 1));
+  HIP_TRY(hipMemcpyAsync(bctl, L.pinned, sz_lay + sz_nb, hipMemcpyHostToDevice, s));
+  launch_fill_u32((uint32_t*)d_tot, 0u, sz_tot / 4, s);
+  timer.mark("setup_b");
+  launch_fill_u32(na.neighbors, NONE, 27 * N, s);
+  for (uint32_t l = 1; l < NL; ++l)
+    launch_emit_level_nodes<K>((int)l, kl, lv[l].elem, h_counts[l - 1], lv[l + 1].elem,
+                               lv[l + 1].elem_group, lv[l + 1].first_child, h_counts[l], d_spans,
+                               NL, d_state, d_lay, d_par, na, s);
+  {
+    std::vector<void*> lp(NL + 1, nullptr);
+    for (uint32_t l = 1; l <= NL; ++l) lp[l - 1] = lv[l].elem;
+    launch_emit_roots<K>(kl, lp.data(), n, d_spans, NL, d_state, d_lay, d_par, sem, na, s);
+  }
+  launch_copy_voxels<K>(kl, lv[1].elem, U1, out->voxels, cbytes, s);
+  timer.mark("nodes");
+  if (sem == SVO_SEM_INTENDED)
+    for (uint32_t l = NL - 1; l >= 1; --l)
+      launch_neighbors_level<K>((int)l, kl, lv[l + 1].elem, lv[l + 1].elem_group, h_counts[l],
+                                d_spans, NL, d_state, d_lay, na, s);
+  timer.mark("neighbors");
+  void* attr_tmp = nullptr;
+  if (want_attrs && N) {
+    SVO_TRY(W.get("attr_tmp", attr_tmp_bytes(C.tuning.scan_blocks), &attr_tmp));
+    AttrOut ao{out->verts, out->edges, out->faces, d_tot};
+    launch_attrs(sem, na, N, d_nbase, n, grid_for(N, 2048, C.tuning.scan_blocks), attr_tmp, ao,
+                 s);
+  }
+  timer.mark("attrs");
+  uint64_t* h_tot = (uint64_t*)L.pinned;
+  HIP_TRY(hipMemcpyAsync(h_tot, d_tot, 8 * (4 + 3 * n), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipGetLastError());
+  timer.collect();
+
+  // ---- trees
+  std::vector<uint64_t> tot(h_tot, h_tot + 4 + 3 * n);
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t src = plan.idx[i];
+    statuses[src] = mstat[i];
+    if (mstat[i] != SVO_OK) continue;
+    svo_tree* t = new svo_tree();
+    t->out = out;
+    t->ctx = &C;
+    t->node_base = node_base[i];
+    t->semantics = sem;
+    t->key_bits_out = in[src].params.key_bits ? (int)in[src].params.key_bits
+                                               : (plan.code_bytes == 8 ? 64 : 32);
+    int kbv = 32;
+    validate_params(&in[src].desc, &in[src].params, &kbv);
+    t->key_bits_out = kbv;
+    svo_tree_info& I = t->info;
+    I.depth = (int32_t)lay[i].depth;
+    I.key_bits = kbv;
+    I.semantics = sem;
+    const MeshParams& P = h_par[i];
+    for (int a = 0; a < 3; ++a) {
+      I.res[a] = P.res[a];
+      I.bbox_min[a] = P.bmin[a];
+      I.bbox_max[a] = P.bmax[a];
+      I.bbox_width[a] = P.width[a];
+      I.unit[a] = P.unit[a];
+    }
+    const LevelSpan& s1 = h_spans[(uint64_t)i * NL];
+    t->vox_base = s1.elem_first;
+    I.n_voxels = s1.elem_end - s1.elem_first;
+    I.n_nodes = lay[i].n_nodes;
+    for (uint32_t l = 0; l < lay[i].depth && l < 32; ++l)
+      I.level_nodes[l] = (l + 1 < lay[i].depth) ? lay[i].level_off[l + 1] - lay[i].level_off[l] : 1;
+    if (want_attrs) {
+      // records of mesh i run from its first node's offsets to the next mesh's
+      uint64_t nv = tot[0], ne = tot[1], nf = tot[2];
+      for (uint32_t j = i + 1; j < n; ++j)
+        if (mstat[j] == SVO_OK && lay[j].n_nodes) {
+          nv = tot[4 + 3 * j];
+          ne = tot[4 + 3 * j + 1];
+          nf = tot[4 + 3 * j + 2];
+          break;
+        }
+      t->v_base = tot[4 + 3 * i];
+      t->e_base = tot[4 + 3 * i + 1];
+      t->f_base = tot[4 + 3 * i + 2];
+      I.n_vertices = nv - t->v_base;
+      I.n_edges = ne - t->e_base;
+      I.n_faces = nf - t->f_base;
+    }
+    I.n_hits = n == 1 ? n_hits : 0;
+    I.device_bytes = out->block->bytes;
+    trees[src] = t;
+  }
+  return SVO_OK;
+}
+
+}  // namespace
+
+int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree*>& trees,
+              std::vector<int>& statuses) {
+  trees.assign(in.size(), nullptr);
+  statuses.assign(in.size(), SVO_OK);
+  Plan plan;
+  bool any64 = false;
+  for (uint32_t i = 0; i < in.size(); ++i) {
+    int kb = 32;
+    const int st = validate_params(&in[i].desc, &in[i].params, &kb);
+    statuses[i] = st;
+    if (st != SVO_OK) continue;
+    if (!plan.idx.empty() && (in[i].params.semantics != in[plan.idx[0]].params.semantics ||
+                              in[i].params.flags != in[plan.idx[0]].params.flags)) {
+      statuses[i] = SVO_E_INVALID;  // one semantics / flag set per pass
+      continue;
+    }
+    plan.idx.push_back(i);
+    any64 |= kb == 64;
+    for (int a = 0; a < 3; ++a) plan.bits = std::max(plan.bits, bits_for(in[i].params.res[a]));
+  }
+  if (plan.idx.empty()) return SVO_OK;
+  plan.cb = 3 * plan.bits;
+  plan.mb = bits_for((int32_t)plan.idx.size());
+  plan.n_levels = plan.bits + 1;
+  if (plan.cb + plan.mb > 64) return SVO_E_RES_LIMIT;
+  plan.k64 = plan.cb + plan.mb > 32;
+  plan.code_bytes = any64 ? 8 : 4;
+  Lane* L = C.acquire_lane();
+  if (!L) return SVO_E_HIP;
+  hipSetDevice(C.device);
+  int st = plan.k64 ? run_typed<uint64_t>(C, *L, in, plan, trees, statuses)
+                    : run_typed<uint32_t>(C, *L, in, plan, trees, statuses);
+  if (st != SVO_OK) {
+    hipStreamSynchronize(L->stream);
+    hipGetLastError();
+    for (auto& t : trees) {
+      delete t;
+      t = nullptr;
+    }
+    for (uint32_t i : plan.idx) statuses[i] = st;
+  }
+  C.release_lane(L);
+  return st;
+}
+
+}  // namespace svo

@@ -1,0 +1,214 @@
+// serve.cpp -- continuous batching of build requests on one device.
+//
+// Requests are queued by svo_server_submit.  n_lanes worker threads each own
+// a pipeline lane (a HIP stream with its own workspace).  A free worker forms
+// a batch from the head of the queue: it waits up to max_wait_us after the
+// oldest pending request for more to arrive, then admits requests in arrival
+// order while their host-side memory estimates fit the budget together with
+// everything already resident or in flight.  Batches built by different
+// workers run concurrently on the device.  Finished trees stay resident until
+// the caller frees them; the budget check sees that through the context's
+// byte counter.  A batch that still runs out of memory goes back to the head
+// of the queue and waits for memory to be released, unless nothing else holds
+// memory, in which case its requests fail with SVO_E_OOM.
+#include <chrono>
+#include <deque>
+#include <thread>
+
+#include "svo_host.h"
+
+using namespace svo;
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct Request {
+  uint64_t id;
+  std::vector<float> pts;
+  std::vector<int32_t> tris;
+  svo_params params;
+  uint64_t estimate;
+  double t_submit;
+};
+
+}  // namespace
+
+struct svo_server {
+  Context* ctx = nullptr;
+  svo_server_cfg cfg{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request> pending;
+  std::deque<svo_completion> done;
+  std::vector<std::thread> workers;
+  bool stop = false;
+  uint64_t next_id = 1;
+  uint64_t in_flight_bytes = 0;  // estimates of batches being built
+  uint32_t in_flight = 0;
+
+  void worker_loop();
+};
+
+void svo_server::worker_loop() {
+  std::unique_lock<std::mutex> lk(mu);
+  while (true) {
+    cv_work.wait(lk, [&] { return stop || !pending.empty(); });
+    if (stop && pending.empty()) return;
+    // batching window measured from the oldest pending request
+    const double deadline = pending.front().t_submit + cfg.max_wait_us * 1e-6;
+    while (!stop && pending.size() < cfg.max_batch) {
+      const double left = deadline - now_s();
+      if (left <= 0) break;
+      cv_work.wait_for(lk, std::chrono::duration<double>(left));
+      if (pending.empty()) break;
+    }
+    if (pending.empty()) continue;
+    // admission against the budget
+    std::vector<Request> batch;
+    uint64_t est = 0;
+    const uint64_t resident = ctx->in_use.load();
+    while (!pending.empty() && batch.size() < cfg.max_batch) {
+      const uint64_t e = pending.front().estimate;
+      if (cfg.mem_budget && resident + in_flight_bytes + est + e > cfg.mem_budget &&
+          !batch.empty())
+        break;
+      if (cfg.mem_budget && resident + in_flight_bytes + est + e > cfg.mem_budget &&
+          batch.empty() && in_flight > 0) {
+        break;  // wait for the batches in flight
+      }
+      est += e;
+      batch.push_back(std::move(pending.front()));
+      pending.pop_front();
+    }
+    if (batch.empty()) {
+      cv_work.wait_for(lk, std::chrono::microseconds(200));
+      continue;
+    }
+    in_flight_bytes += est;
+    ++in_flight;
+    lk.unlock();
+    const double t_start = now_s();
+    std::vector<MeshInput> in(batch.size());
+    for (size_t i = 0; i < batch.size(); ++i) {
+      in[i].desc.xyz = batch[i].pts.data();
+      in[i].desc.n_points = batch[i].pts.size() / 3;
+      in[i].desc.tri = batch[i].tris.data();
+      in[i].desc.n_tris = batch[i].tris.size() / 3;
+      in[i].params = batch[i].params;
+      in[i].on_device = false;
+    }
+    std::vector<svo_tree*> trees;
+    std::vector<int> st;
+    const int s = run_batch(*ctx, in, trees, st);
+    const double t_done = now_s();
+    lk.lock();
+    in_flight_bytes -= est;
+    --in_flight;
+    // out of memory: retry for up to a second while other work or resident
+    // trees may release memory, then report SVO_E_OOM
+    if (s == SVO_E_OOM && !stop && now_s() - batch.front().t_submit < 1.0) {
+      for (auto it = batch.rbegin(); it != batch.rend(); ++it) pending.push_front(std::move(*it));
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(200));
+      lk.lock();
+      continue;
+    }
+    for (size_t i = 0; i < batch.size(); ++i) {
+      svo_completion c{};
+      c.request_id = batch[i].id;
+      c.status = s != SVO_OK ? s : st[i];
+      c.batch_size = (uint32_t)batch.size();
+      c.tree = s == SVO_OK ? trees[i] : nullptr;
+      c.t_submit_s = batch[i].t_submit;
+      c.t_start_s = t_start;
+      c.t_done_s = t_done;
+      done.push_back(c);
+    }
+    cv_done.notify_all();
+    cv_work.notify_all();
+  }
+}
+
+extern "C" {
+
+int svo_server_create(svo_ctx* ctx, const svo_server_cfg* cfg, svo_server** out) {
+  if (!ctx || !cfg || !out) return SVO_E_INVALID;
+  svo_server* s = new svo_server();
+  s->ctx = &ctx->impl;
+  s->cfg = *cfg;
+  if (!s->cfg.max_batch) s->cfg.max_batch = 64;
+  if (!s->cfg.max_queue) s->cfg.max_queue = 1 << 16;
+  if (!s->cfg.n_lanes) s->cfg.n_lanes = 2;
+  s->cfg.n_lanes = std::min<uint32_t>(s->cfg.n_lanes, 4);
+  for (uint32_t i = 0; i < s->cfg.n_lanes; ++i) s->workers.emplace_back([s] { s->worker_loop(); });
+  *out = s;
+  return SVO_OK;
+}
+
+int svo_server_submit(svo_server* srv, const svo_mesh_desc* mesh, const svo_params* params,
+                      uint64_t* request_id) {
+  if (!srv || !mesh || !params) return SVO_E_INVALID;
+  const int v = validate_params(mesh, params, nullptr);
+  if (v != SVO_OK) return v;
+  Request r;
+  r.pts.assign(mesh->xyz, mesh->xyz + 3 * mesh->n_points);
+  r.tris.assign(mesh->tri, mesh->tri + 3 * mesh->n_tris);
+  r.params = *params;
+  r.estimate = estimate_bytes(mesh->n_points, mesh->n_tris, params);
+  r.t_submit = now_s();
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (srv->pending.size() >= srv->cfg.max_queue) return SVO_E_BUSY;
+    r.id = srv->next_id++;
+    if (request_id) *request_id = r.id;
+    srv->pending.push_back(std::move(r));
+  }
+  srv->cv_work.notify_one();
+  return SVO_OK;
+}
+
+int svo_server_poll(svo_server* srv, svo_completion* out, uint32_t cap, uint32_t* n,
+                    int64_t timeout_us) {
+  if (!srv || !n || (cap && !out)) return SVO_E_INVALID;
+  std::unique_lock<std::mutex> lk(srv->mu);
+  if (srv->done.empty() && timeout_us > 0)
+    srv->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us),
+                          [&] { return !srv->done.empty(); });
+  uint32_t k = 0;
+  while (k < cap && !srv->done.empty()) {
+    out[k++] = srv->done.front();
+    srv->done.pop_front();
+  }
+  *n = k;
+  return SVO_OK;
+}
+
+int svo_server_stats(svo_server* srv, uint64_t* bytes_reserved, uint32_t* pending,
+                     uint32_t* in_flight) {
+  if (!srv) return SVO_E_INVALID;
+  std::lock_guard<std::mutex> lk(srv->mu);
+  if (bytes_reserved) *bytes_reserved = srv->in_flight_bytes + srv->ctx->in_use.load();
+  if (pending) *pending = (uint32_t)srv->pending.size();
+  if (in_flight) *in_flight = srv->in_flight;
+  return SVO_OK;
+}
+
+int svo_server_destroy(svo_server* srv) {
+  if (!srv) return SVO_E_INVALID;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    srv->stop = true;
+  }
+  srv->cv_work.notify_all();
+  for (auto& t : srv->workers) t.join();
+  for (auto& c : srv->done)
+    if (c.tree) svo_tree_free(c.tree);
+  delete srv;
+  return SVO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+// svo_host.h -- host-side objects behind the C ABI: context, pipeline lanes,
+// workspaces, batch outputs and trees.
+#pragma once
+#include <atomic>
+#include <condition_variable>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "svo_internal.h"
+
+namespace svo {
+
+struct Context;
+
+// Grow-only device buffers of one pipeline lane.
+struct Workspace {
+  struct Buf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+  };
+  Context* ctx = nullptr;
+  std::unordered_map<std::string, Buf> bufs;
+  int get(const char* name, size_t bytes, void** out);
+  void release_all();
+  size_t total() const;
+};
+
+// One stream and its workspace; one build runs on a lane at a time.
+struct Lane {
+  hipStream_t stream = nullptr;
+  Workspace ws;
+  void* pinned = nullptr;  // host readback area
+  size_t pinned_bytes = 0;
+  std::vector<hipEvent_t> events;
+  int ensure_pinned(size_t bytes);
+};
+
+// Refcounted device allocation holding a batch's output arrays.
+struct DeviceBlock {
+  Context* ctx = nullptr;
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  ~DeviceBlock();
+};
+
+struct BatchOutput {
+  std::shared_ptr<DeviceBlock> block;
+  NodeArrays nodes{};
+  svo_vertex* verts = nullptr;
+  svo_edge* edges = nullptr;
+  svo_edge* faces = nullptr;
+  void* voxels = nullptr;
+  int code_bytes = 4;
+};
+
+struct Context {
+  int device = 0;
+  uint64_t budget = 0;  // 0 = unbounded
+  std::atomic<uint64_t> in_use{0};
+  Tuning tuning;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<std::unique_ptr<Lane>> lanes;
+  std::vector<Lane*> free_lanes;
+  hipStream_t aux_stream = nullptr;  // copies for read-out
+  std::mutex aux_mu;
+  bool timing = false;
+  std::mutex stage_mu;
+  std::vector<std::string> stage_names;
+  std::vector<double> stage_ms;
+
+  int init(int dev, uint64_t budget);
+  void destroy();
+  Lane* acquire_lane();
+  void release_lane(Lane* l);
+  int reserve(uint64_t bytes);  // budget accounting; SVO_E_OOM if over
+  void release(uint64_t bytes);
+  int dev_malloc(void** p, size_t bytes);
+  void dev_free(void* p, size_t bytes);
+};
+
+}  // namespace svo
+
+struct svo_ctx {
+  svo::Context impl;
+};
+
+struct svo_mesh {
+  std::vector<float> points;
+  std::vector<int32_t> tris;
+};
+
+struct svo_tree {
+  std::shared_ptr<svo::BatchOutput> out;
+  svo::Context* ctx = nullptr;
+  svo_tree_info info{};
+  uint64_t node_base = 0;
+  uint64_t vox_base = 0;
+  uint64_t v_base = 0, e_base = 0, f_base = 0;
+  int key_bits_out = 32;
+  int semantics = 0;
+};
+
+namespace svo {
+
+struct MeshInput {
+  svo_mesh_desc desc;
+  svo_params params;
+  bool on_device;
+};
+
+// Runs the pipeline over the meshes.  trees[i] receives a tree or nullptr;
+// statuses[i] the per-mesh status.  Returns SVO_OK unless the pass itself
+// failed (HIP error or out of memory).
+int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree*>& trees,
+              std::vector<int>& statuses);
+
+int validate_params(const svo_mesh_desc* mesh, const svo_params* p, int* key_bits_out);
+uint64_t estimate_bytes(uint64_t n_points, uint64_t n_tris, const svo_params* p);
+
+// OBJ I/O (obj_io.cpp).
+int load_obj_file(const char* path, std::vector<float>& pts, std::vector<int32_t>& tris);
+int write_tree_obj(const char* path, const float* origin, const float* width, uint64_t n);
+
+}  // namespace svo

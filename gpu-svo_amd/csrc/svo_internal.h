@@ -1,0 +1,135 @@
+// svo_internal.h -- host-side declarations shared by the pipeline and the
+// kernel translation units.  Every launcher takes device pointers and a
+// stream and enqueues work only; none of them synchronises.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "svo_device.h"
+
+namespace svo {
+
+// ------------------------------------------------------------ tuning knobs
+struct Tuning {
+  uint32_t small_max_cells = 512;   // bbox cells handled by one lane
+  uint32_t chunk_cells = 4096;      // cells per big-triangle work item
+  uint32_t sort_blocks = 1024;      // fixed grid of the radix sort
+  uint32_t scan_blocks = 1024;      // fixed grid of the level / attribute scans
+};
+
+// ------------------------------------------------------------ k_voxelize.hip
+// Per-mesh ordered-float min/max of all points (6 x uint32 per mesh) and the
+// grid parameters derived from it (BaseModel.h:24-73, SVO.cu:342-351).
+void launch_bbox(const MeshDev* meshes, uint32_t n_meshes, uint64_t max_points, uint32_t* acc,
+                 hipStream_t s);
+void launch_params(const MeshDev* meshes, uint32_t n_meshes, const uint32_t* acc,
+                   MeshParams* params, hipStream_t s);
+// Candidate cells and big-triangle chunk counts (2 x uint64 in `totals`).
+void launch_classify(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
+                     uint32_t n_meshes, uint64_t n_tris, const Tuning& tn,
+                     unsigned long long* totals, hipStream_t s);
+// Surface voxelization (SVO.cu:37-188): emits composite keys.
+template <typename K>
+void launch_voxelize(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
+                     uint32_t n_meshes, uint64_t n_tris, int cb, const Tuning& tn, K* keys,
+                     unsigned long long* key_count, BigItem* big, unsigned long long* big_count,
+                     uint64_t big_cap, hipStream_t s);
+
+// ------------------------------------------------------------ k_sort.hip
+// LSD radix sort of n (device-resident count) keys on bits [0, bits).
+// Returns in *result which of (a, b) holds the sorted keys.  `tmp` needs
+// radix_sort_tmp_bytes(blocks) bytes.
+size_t radix_sort_tmp_bytes(uint32_t blocks);
+template <typename K>
+void launch_radix_sort(K* a, K* b, const unsigned long long* n, uint64_t cap, int bits,
+                       uint32_t blocks, void* tmp, K** result, hipStream_t s);
+
+// ------------------------------------------------------------ k_levels.hip
+struct LevelArrays {
+  void* elem;            // K[cap]: sorted unique keys of this level (S_l)
+  uint32_t* elem_group;  // [cap]: group index (at this level) of each element
+  uint32_t* first_child; // [cap]: for groups of the level below: first member index
+  uint64_t cap;
+};
+// Level 1 from the sorted hit keys: unique codes, their groups, and the
+// level-2 element array (parents).  Counts go to counts[0] (level 1 elements)
+// and counts[1] (level 2 elements).
+template <typename K>
+void launch_level_first(const K* sorted, const unsigned long long* n_sorted, KeyLayout<K> kl,
+                        LevelArrays l1, LevelArrays l2, unsigned long long* counts,
+                        LevelSpan* spans1, uint32_t n_levels, uint32_t blocks, void* tmp,
+                        hipStream_t s);
+// Marks meshes whose level-1 set holds a single voxel as depth-1 trees.
+void launch_level_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint32_t n_levels,
+                                 MeshTreeState* state, hipStream_t s);
+// One level l >= 2: roots of meshes whose level holds one element; groups of
+// the others feed level l+1.
+template <typename K>
+void launch_level(int level, KeyLayout<K> kl, LevelArrays cur, LevelArrays next,
+                  unsigned long long* counts, LevelSpan* spans, uint32_t n_meshes,
+                  uint32_t n_levels, MeshTreeState* state, uint32_t blocks, void* tmp,
+                  hipStream_t s);
+size_t level_tmp_bytes(uint32_t blocks);
+
+// ------------------------------------------------------------ k_attrs.hip
+// Per mesh node layout computed on the host after the level phase.
+struct MeshNodeLayout {
+  uint64_t node_base;    // first node of the mesh in the batch arrays
+  uint64_t n_nodes;
+  uint32_t depth;
+  uint32_t key_out_bits; // 32 or 64
+  uint64_t level_off[24];// node offset of each level within the mesh (level l at [l-1])
+  float width[24];       // node width per level
+};
+
+struct NodeArrays {
+  void* code;            // uint32 or uint64 per node (code_bytes)
+  uint8_t* is_leaf;
+  float* origin;         // 3 per node
+  float* width;
+  uint32_t* parent;
+  uint32_t* childs;      // 8 per node
+  uint32_t* neighbors;   // 27 per node
+  int code_bytes;
+};
+
+template <typename K>
+void launch_emit_level_nodes(int level, KeyLayout<K> kl, const void* elem_l, uint64_t n_elem_l,
+                             const void* elem_next, const uint32_t* elem_group_next,
+                             const uint32_t* first_child_next, uint64_t n_groups,
+                             const LevelSpan* spans, uint32_t n_levels,
+                             const MeshTreeState* state, const MeshNodeLayout* layout,
+                             const MeshParams* params, NodeArrays out, hipStream_t s);
+template <typename K>
+void launch_emit_roots(KeyLayout<K> kl, void* const* level_elems, uint32_t n_meshes,
+                       const LevelSpan* spans, uint32_t n_levels, const MeshTreeState* state,
+                       const MeshNodeLayout* layout, const MeshParams* params, int semantics,
+                       NodeArrays out, hipStream_t s);
+template <typename K>
+void launch_copy_voxels(KeyLayout<K> kl, const void* elem1, uint64_t n, void* out,
+                        int out_bytes, hipStream_t s);
+void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+template <typename K>
+void launch_neighbors_level(int level, KeyLayout<K> kl, const void* elem_next,
+                            const uint32_t* elem_group_next, uint64_t n_groups,
+                            const LevelSpan* spans, uint32_t n_levels,
+                            const MeshTreeState* state, const MeshNodeLayout* layout,
+                            NodeArrays nodes, hipStream_t s);
+// Vertex / edge / face records: count, scan, write.
+struct AttrOut {
+  svo_vertex_t* verts;
+  svo_edge_t* edges;
+  svo_edge_t* faces;
+  unsigned long long* totals;  // [3] totals after the write
+};
+size_t attr_tmp_bytes(uint32_t blocks);
+void launch_attrs(int semantics, const NodeArrays& nodes, uint64_t n_nodes,
+                  const uint64_t* mesh_node_base, uint32_t n_meshes, uint32_t blocks, void* tmp,
+                  AttrOut out, hipStream_t s);
+// 168-byte SVONode records for nodes [first, first + n) (32-bit codes).
+void launch_pack_aos(const NodeArrays& nodes, uint64_t first, uint64_t n, void* dst,
+                     hipStream_t s);
+void launch_query_neighbors(const uint32_t* neighbors, uint64_t n_nodes, const uint32_t* node,
+                            const uint8_t* slot, uint64_t n, uint32_t* out, hipStream_t s);
+
+}  // namespace svo

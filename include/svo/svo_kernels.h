@@ -1,0 +1,35 @@
+/*
+ * svo_kernels.h -- kernel-level entry points of libsvo_amd.so, for tests and
+ * for callers that bring their own keys.  Host buffers in and out; the
+ * library stages them through device memory and runs the same gfx950
+ * kernels the build uses.
+ *
+ *   svo_k_radix_sort    LSD radix sort of Morton keys; in the reference the
+ *                       sorted unique codes come from a dense flag array +
+ *                       thrust::exclusive_scan + compactArray
+ *                       (SVO.cu:367-405, 208-217).
+ *   svo_k_voxelize      surface voxelization of one mesh (surfaceVoxelize,
+ *                       SVO.cu:37-188): the raw hit codes.
+ */
+#ifndef SVO_SVO_KERNELS_H
+#define SVO_SVO_KERNELS_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Sorts n keys (key_bytes 4 or 8) on bits [0, bits) in place. */
+int svo_k_radix_sort(int device, void* keys, uint64_t n, int key_bytes, int bits);
+
+/* Voxelizes one mesh: writes up to cap hit codes (uint32 for key_bits 32,
+ * uint64 for 64) in emission order, duplicates included; n_hits receives the
+ * number of hits (may exceed cap). */
+int svo_k_voxelize(int device, const float* xyz, uint64_t n_points, const int32_t* tri,
+                   uint64_t n_tris, const int32_t* res, uint32_t key_bits, void* codes,
+                   uint64_t cap, uint64_t* n_hits);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SVO_SVO_KERNELS_H */

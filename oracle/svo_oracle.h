@@ -1,0 +1,109 @@
+/*
+ * svo_oracle.h -- CPU restatement of the Alan-Leo-Wong/GPU-SVO build path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness oracle for the
+ * HIP implementation in gpu-svo_amd/.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it, and only as the checker or as
+ * the reported CPU baseline -- never as the thing measured or shipped.
+ *
+ * It restates, in plain C++17 without Eigen or Thrust:
+ *   BaseModel.cpp:3-25      OBJ parse                     -> oracle_load_obj
+ *   BaseModel.h:24-73       bbox cube + 1/10001 padding   -> oracle_build
+ *   SVO.cu:37-188           surfaceVoxelize               -> oracle_build
+ *   SVO.cu:208-520          compaction + level loop       -> oracle_build
+ *   SVO.cu:522-565          neighbors                     -> oracle_build
+ *   SVO.cu:567-728          vertices / edges + unique     -> oracle_build
+ *   SVO.cu:737-753, IO.h    <stem>_<depth>_tree.obj       -> oracle_write_obj
+ *   MortonLUT.h             encode / decode / neighbor LUTs
+ * Floating point follows the reference's operation order with no contraction
+ * (built with -ffp-contract=off).
+ */
+#ifndef SVO_ORACLE_H
+#define SVO_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: identical numbering to the product library (include/svo/svo.h). */
+enum {
+  ORC_OK = 0,
+  ORC_E_INVALID = -1,
+  ORC_E_RES_LIMIT = -2,
+  ORC_E_OOM = -3,
+  ORC_E_IO = -5,
+  ORC_E_PARSE = -6,
+  ORC_E_EMPTY = -7,
+};
+
+enum { ORC_SEM_REFERENCE = 0, ORC_SEM_INTENDED = 1 };
+
+/* Field ids: identical numbering to SVO_FIELD_* of the product library. */
+enum {
+  ORC_F_VOXELS = 0,     /* uint32 or uint64 Morton codes, sorted unique        */
+  ORC_F_CODE = 1,       /* per node, uint32 or uint64                          */
+  ORC_F_ISLEAF = 2,     /* per node, uint8                                     */
+  ORC_F_ORIGIN = 3,     /* per node, float32 x3                                */
+  ORC_F_WIDTH = 4,      /* per node, float32                                   */
+  ORC_F_PARENT = 5,     /* per node, uint32                                    */
+  ORC_F_CHILDS = 6,     /* per node, uint32 x8                                 */
+  ORC_F_NEIGHBORS = 7,  /* per node, uint32 x27                                */
+  ORC_F_VERTICES = 8,   /* 16-byte records {float3 p; uint32 owner}            */
+  ORC_F_EDGES = 9,      /* 28-byte records {float3 a; float3 b; uint32 owner}  */
+  ORC_F_FACES = 10,     /* 28-byte records {float3 lo; float3 hi; uint32 owner}*/
+  ORC_F_LEVEL_NODES = 11, /* uint64 per level, finest level first              */
+  ORC_F_NODES_AOS = 12, /* 168-byte SVONode records (SVO.h:20-35 layout)        */
+  ORC_F_COUNT = 13
+};
+
+typedef struct {
+  int32_t status;
+  int32_t depth;
+  int32_t key_bits;
+  int32_t semantics;
+  int32_t res[3];
+  int32_t pad0;
+  float bbox_min[3];
+  float bbox_max[3];
+  float bbox_width[3];
+  float unit[3];
+  uint64_t n_voxels;
+  uint64_t n_nodes;
+  uint64_t n_vertices;
+  uint64_t n_edges;
+  uint64_t n_faces;
+  uint64_t level_nodes[32];
+  double build_seconds; /* voxelize -> attributes, excluding parse and write */
+} oracle_info;
+
+int oracle_load_obj(const char* path, float** points, uint64_t* n_points, int32_t** tris,
+                    uint64_t* n_tris);
+void oracle_free_buf(void* p);
+
+/* key_bits: 32 (reference uint32 codes, res <= 1024) or 64 (widened restatement). */
+void* oracle_build(const float* points, uint64_t n_points, const int32_t* tris, uint64_t n_tris,
+                   const int32_t* res, int32_t semantics, int32_t key_bits, int32_t n_threads,
+                   int32_t* status);
+void oracle_tree_free(void* tree);
+int oracle_tree_info(const void* tree, oracle_info* out);
+uint64_t oracle_tree_field_bytes(const void* tree, int32_t field);
+int oracle_tree_get_field(const void* tree, int32_t field, void* dst, uint64_t dst_bytes);
+int oracle_write_obj(const void* tree, const char* path);
+
+/* Building blocks exposed for closed-form tests. */
+uint32_t oracle_morton_encode_u16(uint32_t x, uint32_t y, uint32_t z); /* MortonLUT.h:482-501 */
+uint64_t oracle_morton_encode_u64(uint32_t x, uint32_t y, uint32_t z);
+void oracle_morton_decode(uint64_t code, uint32_t* x, uint32_t* y, uint32_t* z); /* 503-529 */
+void oracle_luts(uint32_t* enc_x256, uint32_t* enc_y256, uint32_t* enc_z256, uint8_t* dec_x512,
+                 uint8_t* dec_y512, uint8_t* dec_z512, int16_t* nb_parent_8x27,
+                 int16_t* nb_child_8x27, int16_t* vert_shared_8x8, int16_t* edge_shared_12x4);
+/* Voxelize only (sorted unique codes); returns count, writes up to cap codes. */
+uint64_t oracle_voxelize(const float* points, uint64_t n_points, const int32_t* tris,
+                         uint64_t n_tris, const int32_t* res, int32_t key_bits, uint64_t* dst,
+                         uint64_t cap, int32_t* status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

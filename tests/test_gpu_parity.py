@@ -1,0 +1,118 @@
+"""HIP build vs the CPU oracle, every output field bit-exact.
+
+Cases cover the reference's sample mesh (bunny.obj), the plumbing config
+(icosphere subdivision 4 at res 64), tori, triangle soups, non-power-of-two
+and non-cubic grids, the widened 64-bit key path, and the failure statuses.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from parity import compare
+
+pytestmark = pytest.mark.gpu
+
+
+def _meshgen():
+    from svo_amd import meshgen
+    return meshgen
+
+
+def _case(ctx, P, T, res, sem=0, kb=32):
+    print(f"case: {P.shape[0]} points {T.shape[0]} tris res {res} sem {sem} key_bits {kb}",
+          flush=True)
+    tree = ctx.build(P, T, res, semantics=sem, key_bits=kb)
+    ref = po.build(P, T, res, semantics=sem, key_bits=kb)
+    assert ref.status == 0
+    compare(tree, ref)
+    return tree, ref
+
+
+@pytest.mark.parametrize("sem", [0, 1])
+@pytest.mark.parametrize("res", [64, 32, 100])
+def test_bunny(gpu_ctx, bunny, res, sem):
+    P, T = bunny
+    _case(gpu_ctx, P, T, res, sem)
+
+
+def test_bunny_noncubic(gpu_ctx, bunny):
+    P, T = bunny
+    _case(gpu_ctx, P, T, (64, 32, 48))
+    _case(gpu_ctx, P, T, (17, 90, 33), sem=1)
+
+
+@pytest.mark.parametrize("sem", [0, 1])
+def test_icosphere4_res64(gpu_ctx, sem):
+    P, T = _meshgen().icosphere(4)
+    tree, ref = _case(gpu_ctx, P, T, 64, sem)
+    assert T.shape[0] == 5120 and P.shape[0] == 2562
+
+
+@pytest.mark.parametrize("res", [128, 256])
+def test_torus(gpu_ctx, res):
+    P, T = _meshgen().torus(0.4, 0.12, 96, 48)
+    _case(gpu_ctx, P, T, res)
+
+
+@pytest.mark.parametrize("seed,n,res", [(11, 500, 64), (12, 3000, 256), (13, 200, 1024)])
+def test_soup(gpu_ctx, seed, n, res):
+    P, T = _meshgen().soup(n, seed=seed, median=1.0 / 64)
+    _case(gpu_ctx, P, T, res)
+    _case(gpu_ctx, P, T, res, sem=1)
+
+
+def test_big_triangles(gpu_ctx):
+    # a few triangles spanning most of the grid take the big-triangle path
+    rng = np.random.default_rng(5)
+    P = rng.random((12, 3)).astype(np.float32)
+    T = np.arange(12, dtype=np.int32).reshape(4, 3)
+    _case(gpu_ctx, P, T, 128)
+    _case(gpu_ctx, P, T, 300, sem=1)
+
+
+def _slivers(n, seed):
+    # long thin axis-following triangles: few box cells even at high resolution
+    rng = np.random.default_rng(seed)
+    a = rng.random((n, 3)) * 0.6
+    d = np.zeros((n, 3))
+    d[np.arange(n), rng.integers(0, 3, n)] = 0.3 + 0.1 * rng.random(n)
+    b = a + d + 0.001 * rng.random((n, 3))
+    c = a + 0.002 * rng.random((n, 3))
+    P = np.stack([a, b, c], 1).reshape(-1, 3).astype(np.float32)
+    return P, np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+
+
+def test_key64(gpu_ctx):
+    P, T = _meshgen().icosphere(3)
+    _case(gpu_ctx, P, T, 64, kb=64)
+    P, T = _slivers(20, 3)
+    _case(gpu_ctx, P, T, 2048, kb=64)
+    _case(gpu_ctx, P, T, 4096, kb=64, sem=1)
+
+
+def test_single_triangle(gpu_ctx):
+    P = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0.5]], np.float32)
+    T = np.array([[0, 1, 2]], np.int32)
+    for res in (1, 2, 5, 16):
+        _case(gpu_ctx, P, T, res)
+
+
+def test_statuses(gpu_ctx):
+    import svo_amd
+    P = np.zeros((3, 3), np.float32)
+    T = np.array([[0, 1, 2]], np.int32)
+    with pytest.raises(svo_amd.SvoError) as e:
+        gpu_ctx.build(P, T, 16)  # zero-extent box
+    assert e.value.status == svo_amd.E_INVALID
+    P = np.random.default_rng(0).random((3, 3)).astype(np.float32)
+    with pytest.raises(svo_amd.SvoError) as e:
+        gpu_ctx.build(P, np.array([[0, 1, 3]], np.int32), 16)
+    assert e.value.status == svo_amd.E_INVALID
+    with pytest.raises(svo_amd.SvoError) as e:
+        gpu_ctx.build(P, T, 2048, key_bits=32)
+    assert e.value.status == svo_amd.E_RES_LIMIT
+    P2 = P.copy()
+    P2[1, 0] = np.nan
+    with pytest.raises(svo_amd.SvoError) as e:
+        gpu_ctx.build(P2, T, 16)
+    assert e.value.status == svo_amd.E_INVALID
