@@ -1,0 +1,222 @@
+"""Benchmark of the MI355X SVO builder on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batch|single|soup]
+
+Default workload (the metric's configuration): 256 mixed meshes -- icospheres
+of subdivision 2-6 and tori, 320-81,920 triangles log-uniform, resolution
+drawn from {64, 128, 256} (seeded) -- with the meshes already resident in
+device memory.  A step is one full build of all 256 octrees (voxelize ->
+sorted unique voxels -> levels -> node records -> neighbors -> vertex/edge
+records, reference semantics) in one batched pass; the same meshes built one
+per pass (the plain path) are timed beside it.  Output: one JSON line.
+
+`roofline` is computed for the kernel stage that takes the most device time,
+from HIP events recorded on the build's stream, and its algorithmic bytes.
+`cpu_baseline` times the CPU restatement of the reference (the oracle) on a
+bounded sample of the same meshes on this host, single-threaded.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "gpu-svo_amd"), os.path.join(ROOT, "oracle")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "meshes/sec + voxels/sec at res 256, batched vs plain; p99 latency; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def stage_bytes(stats, name):
+    """Algorithmic HBM bytes of one pipeline stage (read + write, each once)."""
+    kb = stats["key_bytes"]
+    H, U, N = stats["n_hits"], stats["n_voxels"], stats["n_nodes"]
+    if name == "sort":
+        return 2 * kb * H * stats["sort_passes"]
+    if name == "voxelize":
+        return 36 * stats["n_tris"] + kb * H
+    if name == "unique_level1":
+        return kb * H + (kb + 4) * U + 8 * U
+    if name == "levels":
+        return (kb + 4) * stats["level_elems"]
+    if name == "neighbor_fill":
+        return 108 * N
+    if name == "nodes":
+        return 57 * N + 4 * N
+    if name == "attr_write":
+        return 16 * stats["n_vertices"] + 28 * stats["n_edges"] + 28 * stats["n_faces"] + 32 * N
+    if name == "attr_count":
+        return 32 * N
+    return 0
+
+
+def pass_stats(ctx):
+    import ctypes as C
+    import svo_amd
+    L = svo_amd.lib()
+
+    class PassStats(C.Structure):
+        _fields_ = [(k, C.c_uint64) for k in (
+            "n_meshes", "n_tris", "box_cells", "n_hits", "n_voxels", "level_elems", "n_nodes",
+            "n_vertices", "n_edges", "n_faces", "out_bytes")] + \
+            [(k, C.c_uint32) for k in ("key_bytes", "sort_bits", "sort_passes", "n_levels")]
+    fn = L.svo_ctx_last_pass_stats
+    fn.argtypes = [C.c_void_p, C.POINTER(PassStats)]
+    ps = PassStats()
+    fn(ctx.handle, C.byref(ps))
+    return {k: getattr(ps, k) for k, _ in PassStats._fields_}
+
+
+def cpu_baseline(meshes, budget_s=12.0):
+    """Oracle (CPU restatement of the reference) on a bounded sample, 1 thread."""
+    import pyoracle as po
+    done = 0
+    t_build = 0.0
+    for P, T, res in meshes:
+        r = po.build(P, T, res, threads=1, fields=["voxels"])
+        t_build += r.info.build_seconds
+        done += 1
+        if t_build > budget_s:
+            break
+    return {"value": round(done / t_build, 3), "unit": "meshes/s", "cores": 1, "kind": "port",
+            "sample": f"first {done} meshes of the same 256-mesh set, build only "
+                      f"(voxelize -> vertex/edge), {t_build:.1f} s"}
+
+
+def run_batch_workload(args):
+    import numpy as np
+    import svo_amd
+    from svo_amd import meshgen
+
+    log("bench: generating 256 mixed meshes")
+    meshes = meshgen.mixed_batch(256, seed=3)
+    n_tris = sum(int(T.shape[0]) for _, T, _ in meshes)
+    ctx = svo_amd.Context(0)
+    ctx.set_timing(True)
+    dms = [ctx.upload_mesh(P, T) for P, T, _ in meshes]
+    descs = [dm.desc(res) for dm, (_, _, res) in zip(dms, meshes)]
+
+    def batch_step():
+        trees, st = ctx.build_batch(descs, on_device=True)
+        bad = [s for s in st if s != 0]
+        if bad:
+            raise RuntimeError(f"batched build failed: statuses {set(bad)}")
+        nv = sum(t.info.n_voxels for t in trees)
+        nn = sum(t.info.n_nodes for t in trees)
+        for t in trees:
+            t.free()
+        return nv, nn
+
+    def plain_step():
+        nv = nn = 0
+        for d in descs:
+            t = ctx.build_device(*d)
+            nv += t.info.n_voxels
+            nn += t.info.n_nodes
+            t.free()
+        return nv, nn
+
+    for i in range(args.warmup):
+        log(f"bench: warmup batched {i}")
+        batch_step()
+    stage_acc: dict = {}
+    log(f"bench: timing {args.steps} batched steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        nv, nn = batch_step()
+        for k, v in ctx.stage_ms().items():
+            stage_acc[k] = stage_acc.get(k, 0.0) + v
+    t1 = time.perf_counter()
+    stats = pass_stats(ctx)
+    ms_batch = (t1 - t0) * 1e3 / args.steps
+    stages = {k: v / args.steps for k, v in stage_acc.items()}
+
+    log("bench: plain path (one mesh per pass)")
+    plain_step()  # warmup
+    ps = max(1, args.plain_steps)
+    t0 = time.perf_counter()
+    for _ in range(ps):
+        plain_step()
+    t1 = time.perf_counter()
+    ms_plain = (t1 - t0) * 1e3 / ps
+
+    # roofline for the stage with the most device time
+    dev = {k: v for k, v in stages.items() if not k.startswith("setup")}
+    dom = max(dev, key=dev.get)
+    dom_bytes = stage_bytes(stats, dom)
+    achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9 if stages[dom] > 0 else 0.0
+    sort_gbs = stage_bytes(stats, "sort") / (stages["sort"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(256 / (ms_batch * 1e-3), 2),
+        "unit": "meshes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_batch, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32" if stats["key_bytes"] == 4 else "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": "256 mixed meshes (icospheres subdiv 2-6 + tori, 320-81,920 triangles "
+                        "log-uniform, res from {64,128,256}, seed 3), one batched pass per step, "
+                        "reference semantics, inputs resident in HBM",
+            "n_meshes": 256, "n_tris": n_tris, "parallelism": "single GPU",
+        },
+        "voxels_per_s": round(nv / (ms_batch * 1e-3)),
+        "nodes_per_s": round(nn / (ms_batch * 1e-3)),
+        "plain_meshes_per_s": round(256 / (ms_plain * 1e-3), 2),
+        "plain_ms_per_pass_of_256": round(ms_plain, 3),
+        "batched_vs_plain": round(ms_plain / ms_batch, 3),
+        "sizes": {"voxels": nv, "nodes": nn, "hits": stats["n_hits"],
+                  "vertices": stats["n_vertices"], "edges": stats["n_edges"],
+                  "out_gb": round(stats["out_bytes"] / 1e9, 3)},
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+        "sort_gbs": round(sort_gbs, 1),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": dom_bytes},
+    }
+    for dm in dms:
+        dm.free()
+    ctx.close()
+    if not args.no_cpu:
+        log("bench: CPU baseline (oracle, bounded sample)")
+        out["cpu_baseline"] = cpu_baseline(meshes)
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--workload", default="batch", choices=["batch"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              f"(meshes shard across GPUs without exchange, see the design notes)",
+              file=sys.stderr)
+        sys.exit(2)
+    out = run_batch_workload(args)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -124,6 +124,37 @@ int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap,
 
 uint64_t svo_ctx_bytes_in_use(const svo_ctx* ctx) { return ctx ? ctx->impl.in_use.load() : 0; }
 
+int svo_ctx_last_pass_stats(svo_ctx* ctx, svo_pass_stats* out) {
+  if (!ctx || !out) return SVO_E_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->impl.stage_mu);
+  *out = ctx->impl.last_pass;
+  return SVO_OK;
+}
+
+int svo_device_alloc(svo_ctx* ctx, uint64_t bytes, void** out) {
+  if (!ctx || !out) return SVO_E_INVALID;
+  return ctx->impl.dev_malloc(out, bytes);
+}
+
+int svo_device_free(svo_ctx* ctx, void* ptr, uint64_t bytes) {
+  if (!ctx) return SVO_E_INVALID;
+  ctx->impl.dev_free(ptr, bytes);
+  return SVO_OK;
+}
+
+int svo_device_copy(svo_ctx* ctx, void* dst, const void* src, uint64_t bytes, int kind) {
+  if (!ctx || kind < 0 || kind > 2) return SVO_E_INVALID;
+  if (!bytes) return SVO_OK;
+  Context& C = ctx->impl;
+  const hipMemcpyKind k = kind == 0   ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost
+                                      : hipMemcpyDeviceToDevice;
+  std::lock_guard<std::mutex> lk(C.aux_mu);
+  hipSetDevice(C.device);
+  if (hipMemcpyAsync(dst, src, bytes, k, C.aux_stream) != hipSuccess) return SVO_E_HIP;
+  return hipStreamSynchronize(C.aux_stream) == hipSuccess ? SVO_OK : SVO_E_HIP;
+}
+
 int svo_load_obj(const char* path, svo_mesh** out) {
   if (!path || !out) return SVO_E_INVALID;
   *out = nullptr;

@@ -620,14 +620,20 @@ void launch_neighbors_level(int level, KeyLayout<K> kl, const void* elem_next,
 
 size_t attr_tmp_bytes(uint32_t blocks) { return (size_t)3 * blocks * sizeof(uint64_t); }
 
-void launch_attrs(int semantics, const NodeArrays& nodes, uint64_t n_nodes,
-                  const uint64_t* mesh_node_base, uint32_t n_meshes, uint32_t blocks, void* tmp,
-                  AttrOut out, hipStream_t s) {
+void launch_attrs_count(int semantics, const NodeArrays& nodes, uint64_t n_nodes,
+                        const uint64_t* mesh_node_base, uint32_t n_meshes, uint32_t blocks,
+                        void* tmp, AttrOut out, hipStream_t s) {
   uint64_t* partial = (uint64_t*)tmp;
-  uint64_t* mesh_first = (uint64_t*)out.totals + 4;  // [3 * n_meshes] after the totals
   hipLaunchKernelGGL(k_attr_count, dim3(blocks), dim3(BLOCK), 0, s, semantics, nodes, n_nodes,
                      mesh_node_base, n_meshes, blocks, partial);
   hipLaunchKernelGGL(k_triple_scan, dim3(1), dim3(BLOCK), 0, s, partial, blocks, out.totals);
+}
+
+void launch_attrs_write(int semantics, const NodeArrays& nodes, uint64_t n_nodes,
+                        const uint64_t* mesh_node_base, uint32_t n_meshes, uint32_t blocks,
+                        void* tmp, AttrOut out, hipStream_t s) {
+  uint64_t* partial = (uint64_t*)tmp;
+  uint64_t* mesh_first = (uint64_t*)out.totals + 4;  // [3 * n_meshes] after the totals
   hipLaunchKernelGGL(k_attr_write, dim3(blocks), dim3(BLOCK), 0, s, semantics, nodes, n_nodes,
                      mesh_node_base, n_meshes, blocks, (const uint64_t*)partial, out, mesh_first);
 }

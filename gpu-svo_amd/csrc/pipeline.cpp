@@ -547,6 +547,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   launch_fill_u32((uint32_t*)d_tot, 0u, sz_tot / 4, s);
   timer.mark("setup_b");
   launch_fill_u32(na.neighbors, NONE, 27 * N, s);
+  timer.mark("neighbor_fill");
   for (uint32_t l = 1; l < NL; ++l)
     launch_emit_level_nodes<K>((int)l, kl, lv[l].elem, h_counts[l - 1], lv[l + 1].elem,
                                lv[l + 1].elem_group, lv[l + 1].first_child, h_counts[l], d_spans,
@@ -567,10 +568,12 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   if (want_attrs && N) {
     SVO_TRY(W.get("attr_tmp", attr_tmp_bytes(C.tuning.scan_blocks), &attr_tmp));
     AttrOut ao{out->verts, out->edges, out->faces, d_tot};
-    launch_attrs(sem, na, N, d_nbase, n, grid_for(N, 2048, C.tuning.scan_blocks), attr_tmp, ao,
-                 s);
+    const uint32_t ab = grid_for(N, 2048, C.tuning.scan_blocks);
+    launch_attrs_count(sem, na, N, d_nbase, n, ab, attr_tmp, ao, s);
+    timer.mark("attr_count");
+    launch_attrs_write(sem, na, N, d_nbase, n, ab, attr_tmp, ao, s);
   }
-  timer.mark("attrs");
+  timer.mark("attr_write");
   uint64_t* h_tot = (uint64_t*)L.pinned;
   HIP_TRY(hipMemcpyAsync(h_tot, d_tot, 8 * (4 + 3 * n), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -579,6 +582,26 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
 
   // ---- trees
   std::vector<uint64_t> tot(h_tot, h_tot + 4 + 3 * n);
+  {
+    svo_pass_stats ps{};
+    ps.n_meshes = n;
+    ps.n_tris = T;
+    ps.box_cells = cells;
+    ps.n_hits = n_hits;
+    ps.n_voxels = U1;
+    for (uint32_t l = 0; l < NL; ++l) ps.level_elems += h_counts[l];
+    ps.n_nodes = N;
+    ps.n_vertices = tot[0];
+    ps.n_edges = tot[1];
+    ps.n_faces = tot[2];
+    ps.out_bytes = out->block->bytes;
+    ps.key_bytes = sizeof(K);
+    ps.sort_bits = (uint32_t)(plan.cb + plan.mb);
+    ps.sort_passes = (ps.sort_bits + 7) / 8;
+    ps.n_levels = NL;
+    std::lock_guard<std::mutex> lk(C.stage_mu);
+    C.last_pass = ps;
+  }
   for (uint32_t i = 0; i < n; ++i) {
     const uint32_t src = plan.idx[i];
     statuses[src] = mstat[i];

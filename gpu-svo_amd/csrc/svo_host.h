@@ -71,6 +71,7 @@ struct Context {
   std::mutex stage_mu;
   std::vector<std::string> stage_names;
   std::vector<double> stage_ms;
+  svo_pass_stats last_pass{};
 
   int init(int dev, uint64_t budget);
   void destroy();

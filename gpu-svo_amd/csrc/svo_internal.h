@@ -123,9 +123,12 @@ struct AttrOut {
   unsigned long long* totals;  // [3] totals after the write
 };
 size_t attr_tmp_bytes(uint32_t blocks);
-void launch_attrs(int semantics, const NodeArrays& nodes, uint64_t n_nodes,
-                  const uint64_t* mesh_node_base, uint32_t n_meshes, uint32_t blocks, void* tmp,
-                  AttrOut out, hipStream_t s);
+void launch_attrs_count(int semantics, const NodeArrays& nodes, uint64_t n_nodes,
+                        const uint64_t* mesh_node_base, uint32_t n_meshes, uint32_t blocks,
+                        void* tmp, AttrOut out, hipStream_t s);
+void launch_attrs_write(int semantics, const NodeArrays& nodes, uint64_t n_nodes,
+                        const uint64_t* mesh_node_base, uint32_t n_meshes, uint32_t blocks,
+                        void* tmp, AttrOut out, hipStream_t s);
 // 168-byte SVONode records for nodes [first, first + n) (32-bit codes).
 void launch_pack_aos(const NodeArrays& nodes, uint64_t first, uint64_t n, void* dst,
                      hipStream_t s);

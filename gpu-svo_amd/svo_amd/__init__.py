@@ -88,6 +88,9 @@ _SIGS = {
                                         C.POINTER(C.c_char_p), C.c_int,
                                         C.POINTER(C.c_int)]),
     "svo_ctx_bytes_in_use": (C.c_uint64, [C.c_void_p]),
+    "svo_device_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "svo_device_free": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "svo_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]),
     "svo_load_obj": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "svo_mesh_view": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc)]),
     "svo_mesh_free": (None, [C.c_void_p]),
@@ -183,10 +186,14 @@ def load_obj(path: str):
     try:
         d = MeshDesc()
         L.svo_mesh_view(h, C.byref(d))
-        pts = np.ctypeslib.as_array(C.cast(d.xyz, C.POINTER(C.c_float)),
-                                    shape=(max(1, 3 * d.n_points),))[: 3 * d.n_points].copy()
-        tri = np.ctypeslib.as_array(C.cast(d.tri, C.POINTER(C.c_int32)),
-                                    shape=(max(1, 3 * d.n_tris),))[: 3 * d.n_tris].copy()
+        pts = np.zeros(0, np.float32)
+        tri = np.zeros(0, np.int32)
+        if d.n_points:
+            pts = np.ctypeslib.as_array(C.cast(d.xyz, C.POINTER(C.c_float)),
+                                        shape=(3 * d.n_points,)).copy()
+        if d.n_tris:
+            tri = np.ctypeslib.as_array(C.cast(d.tri, C.POINTER(C.c_int32)),
+                                        shape=(3 * d.n_tris,)).copy()
         return pts.reshape(-1, 3), tri.reshape(-1, 3)
     finally:
         L.svo_mesh_free(h)
@@ -343,6 +350,10 @@ class Context:
     def bytes_in_use(self) -> int:
         return int(lib().svo_ctx_bytes_in_use(self._h))
 
+    def upload_mesh(self, points, tris) -> "DeviceMesh":
+        """Copies a mesh to device memory (kept until DeviceMesh.free)."""
+        return DeviceMesh(self, points, tris)
+
     def build(self, points, tris, res, semantics=SEM_REFERENCE, key_bits=0, flags=0) -> Tree:
         d, _keep = _host_desc(points, tris)
         p = make_params(res, semantics, key_bits, flags)
@@ -382,6 +393,33 @@ class Context:
         lib().svo_build_batch(self._h, descs, params, n, 1 if on_device else 0, out, st)
         trees = [Tree(out[i], self) if st[i] == OK and out[i] else None for i in range(n)]
         return trees, [st[i] for i in range(n)]
+
+
+class DeviceMesh:
+    """A mesh resident in device memory of a context."""
+
+    def __init__(self, ctx: Context, points, tris):
+        L = lib()
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        tri = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+        self.ctx = ctx
+        self.n_points, self.n_tris = pts.shape[0], tri.shape[0]
+        self._bytes = (pts.nbytes, tri.nbytes)
+        p, t = C.c_void_p(), C.c_void_p()
+        _check(L.svo_device_alloc(ctx.handle, max(1, pts.nbytes), C.byref(p)), "device alloc")
+        _check(L.svo_device_alloc(ctx.handle, max(1, tri.nbytes), C.byref(t)), "device alloc")
+        self.points_ptr, self.tris_ptr = p.value, t.value
+        _check(L.svo_device_copy(ctx.handle, p, pts.ctypes.data, pts.nbytes, 0), "upload")
+        _check(L.svo_device_copy(ctx.handle, t, tri.ctypes.data, tri.nbytes, 0), "upload")
+
+    def desc(self, res):
+        return (self.points_ptr, self.n_points, self.tris_ptr, self.n_tris, res)
+
+    def free(self):
+        if self.points_ptr:
+            lib().svo_device_free(self.ctx.handle, C.c_void_p(self.points_ptr), max(1, self._bytes[0]))
+            lib().svo_device_free(self.ctx.handle, C.c_void_p(self.tris_ptr), max(1, self._bytes[1]))
+        self.points_ptr = self.tris_ptr = 0
 
 
 @dataclass

@@ -150,7 +150,20 @@ int svo_ctx_destroy(svo_ctx* ctx);
  * measured with HIP events on the build's stream when timing is enabled. */
 int svo_ctx_set_timing(svo_ctx* ctx, int enable);
 int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap, int* n);
+/* Sizes of the last pipeline pass on this context (all meshes of the pass). */
+typedef struct {
+  uint64_t n_meshes, n_tris, box_cells, n_hits, n_voxels, level_elems, n_nodes;
+  uint64_t n_vertices, n_edges, n_faces, out_bytes;
+  uint32_t key_bytes, sort_bits, sort_passes, n_levels;
+} svo_pass_stats;
+int svo_ctx_last_pass_stats(svo_ctx* ctx, svo_pass_stats* out);
 uint64_t svo_ctx_bytes_in_use(const svo_ctx* ctx);
+/* Device memory on the context's device (counted against its budget), for
+ * callers that keep meshes resident for svo_build_device / svo_build_batch. */
+int svo_device_alloc(svo_ctx* ctx, uint64_t bytes, void** out);
+int svo_device_free(svo_ctx* ctx, void* ptr, uint64_t bytes);
+/* kind: 0 = host to device, 1 = device to host, 2 = device to device. */
+int svo_device_copy(svo_ctx* ctx, void* dst, const void* src, uint64_t bytes, int kind);
 
 /* ---- meshes ----------------------------------------------------------- */
 /* BaseModel::loadOBJ rules (BaseModel.cpp:3-25): "v x y z" lines; faces

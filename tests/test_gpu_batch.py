@@ -1,0 +1,93 @@
+"""Batched pass (composite (mesh, Morton) keys) == plain per-mesh builds == oracle."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from parity import compare, assert_same
+
+pytestmark = pytest.mark.gpu
+
+ALL = ["voxels", "code", "is_leaf", "origin", "width", "parent", "childs", "neighbors",
+       "vertices", "edges", "faces", "level_nodes"]
+
+
+def _meshes(n, seed, res_choices=(64, 128)):
+    from svo_amd import meshgen
+    specs = meshgen.mixed_specs(n, seed=seed, res_choices=res_choices)
+    out = []
+    for s in specs:
+        if s.n_tris > 20480:  # keep the oracle cheap
+            continue
+        P, T = s.build()
+        out.append((P, T, s.res))
+    return out
+
+
+@pytest.mark.parametrize("sem", [0, 1])
+def test_batch_equals_plain_and_oracle(gpu_ctx, sem):
+    meshes = _meshes(40, seed=21)
+    print(f"batch of {len(meshes)} meshes, semantics {sem}", flush=True)
+    trees, st = gpu_ctx.build_batch(meshes, semantics=sem)
+    assert all(s == 0 for s in st), st
+    for i, ((P, T, res), tb) in enumerate(zip(meshes, trees)):
+        tp = gpu_ctx.build(P, T, res, semantics=sem)
+        assert tb.depth == tp.depth
+        for f in ALL:
+            assert_same(f"mesh {i} {f}", tb.field(f), tp.field(f))
+        assert_same(f"mesh {i} nodes_aos", tb.field("nodes_aos"), tp.field("nodes_aos"))
+        if i % 5 == 0:
+            compare(tb, po.build(P, T, res, semantics=sem))
+        tp.free()
+    for t in trees:
+        t.free()
+
+
+def test_batch_device_inputs(gpu_ctx):
+    meshes = _meshes(24, seed=22, res_choices=(64, 128, 256))
+    dms = [gpu_ctx.upload_mesh(P, T) for P, T, _ in meshes]
+    try:
+        trees, st = gpu_ctx.build_batch([d.desc(r) for d, (_, _, r) in zip(dms, meshes)],
+                                        on_device=True)
+        assert all(s == 0 for s in st), st
+        for (P, T, res), tb in zip(meshes, trees):
+            tp = gpu_ctx.build(P, T, res)
+            for f in ("voxels", "code", "origin", "parent", "childs", "vertices", "edges"):
+                assert_same(f, tb.field(f), tp.field(f))
+            tp.free()
+            tb.free()
+    finally:
+        for d in dms:
+            d.free()
+
+
+def test_batch_mixed_status(gpu_ctx):
+    import svo_amd
+    from svo_amd import meshgen
+    P, T = meshgen.icosphere(2)
+    flat = np.zeros((3, 3), np.float32)
+    bad_idx = np.array([[0, 1, 999]], np.int32)
+    meshes = [(P, T, 32), (flat, np.array([[0, 1, 2]], np.int32), 32), (P, bad_idx, 32),
+              (P, T, 64), (P, T, 2000)]
+    trees, st = gpu_ctx.build_batch(meshes, key_bits=32)
+    assert st[0] == 0 and st[3] == 0
+    assert st[1] == svo_amd.E_INVALID and st[2] == svo_amd.E_INVALID
+    assert st[4] == svo_amd.E_RES_LIMIT
+    for i in (0, 3):
+        compare(trees[i], po.build(*meshes[i]))
+    for t in trees:
+        if t is not None:
+            t.free()
+
+
+def test_batch_single_voxel_meshes(gpu_ctx):
+    # meshes whose voxel set is a single cell (depth-1 trees) next to normal ones
+    tri = np.array([[0, 1, 2]], np.int32)
+    tiny = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    from svo_amd import meshgen
+    P, T = meshgen.icosphere(3)
+    meshes = [(tiny, tri, 1), (P, T, 16), (tiny, tri, 1), (P, T, 8), (tiny, tri, 2)]
+    trees, st = gpu_ctx.build_batch(meshes)
+    assert all(s == 0 for s in st), st
+    for m, t in zip(meshes, trees):
+        compare(t, po.build(*m))
+        t.free()
