@@ -536,6 +536,89 @@ __global__ __launch_bounds__(BLOCK) void k_attr_write(int semantics, NodeArrays 
   }
 }
 
+// Reference records when no record can equal its predecessor (the host has
+// checked that every width is positive, finite and not absorbed by the
+// coordinates, so all corners of a node are distinct and thrust::unique keeps
+// everything): node t's records sit at the identity offsets 8t and 12t.
+// A block handles 64 consecutive nodes: four lanes per node build two
+// vertices and three edges each into LDS, then the block streams its
+// contiguous 8 KiB of vertex records and 21 KiB of edge records out as
+// aligned 16-byte stores (64 * 336 bytes and 64 * 128 bytes are multiples
+// of 16).  The mesh of the first node is found once per block.
+constexpr int REF_NODES = 64;
+
+__device__ inline void stream_out(const uint32_t* src, uint32_t n_dw, uint32_t* dst) {
+  const uint32_t nq = n_dw >> 2;
+  for (uint32_t q = threadIdx.x; q < nq; q += BLOCK)
+    reinterpret_cast<uint4*>(dst)[q] = reinterpret_cast<const uint4*>(src)[q];
+  for (uint32_t d = (nq << 2) + threadIdx.x; d < n_dw; d += BLOCK) dst[d] = src[d];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_ref_records(const float* origin, const float* width,
+                                                       uint64_t n_nodes, const uint64_t* mbase,
+                                                       uint32_t n_meshes, uint32_t* verts,
+                                                       uint32_t* edges) {
+  __shared__ __align__(16) uint32_t sv[REF_NODES * 32];
+  __shared__ __align__(16) uint32_t se[REF_NODES * 84];
+  __shared__ uint32_t s_mesh;
+  const uint64_t first = (uint64_t)blockIdx.x * REF_NODES;
+  if (first >= n_nodes) return;
+  const uint32_t nn = (uint32_t)((n_nodes - first) < REF_NODES ? (n_nodes - first) : REF_NODES);
+  if (threadIdx.x == 0) s_mesh = find_mesh64(mbase, n_meshes, first);
+  __syncthreads();
+  const uint32_t ln = threadIdx.x >> 2, part = threadIdx.x & 3;
+  if (ln < nn) {
+    const uint64_t t = first + ln;
+    uint32_t m = s_mesh;
+    while (m + 1 < n_meshes && mbase[m + 1] <= t) ++m;
+    const uint32_t self = (uint32_t)(t - mbase[m]);
+    NodeGeom g;
+    g.o[0] = origin[3 * t];
+    g.o[1] = origin[3 * t + 1];
+    g.o[2] = origin[3 * t + 2];
+    g.w = width[t];
+    for (int j = 2 * part; j < 2 * part + 2; ++j) {
+      float p[3];
+      ref_corner(g.w, j, p);
+      uint32_t* d = sv + 32 * ln + 4 * j;
+      d[0] = __float_as_uint(p[0]);
+      d[1] = __float_as_uint(p[1]);
+      d[2] = __float_as_uint(p[2]);
+      d[3] = self;
+    }
+    for (int i = 3 * part; i < 3 * part + 3; ++i) {
+      float a[3], b[3];
+      ref_edge(g, i, a, b);
+      uint32_t* d = se + 84 * ln + 7 * i;
+      d[0] = __float_as_uint(a[0]);
+      d[1] = __float_as_uint(a[1]);
+      d[2] = __float_as_uint(a[2]);
+      d[3] = __float_as_uint(b[0]);
+      d[4] = __float_as_uint(b[1]);
+      d[5] = __float_as_uint(b[2]);
+      d[6] = self;
+    }
+  }
+  __syncthreads();
+  stream_out(sv, 32 * nn, verts + 32 * first);
+  stream_out(se, 84 * nn, edges + 84 * first);
+}
+
+__global__ void k_ref_totals(uint64_t n_nodes, const uint64_t* mbase, uint32_t n_meshes,
+                             unsigned long long* totals) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m == 0) {
+    totals[0] = 8 * n_nodes;
+    totals[1] = 12 * n_nodes;
+    totals[2] = 0;
+  }
+  if (m < n_meshes) {
+    totals[4 + 3 * m] = 8 * mbase[m];
+    totals[4 + 3 * m + 1] = 12 * mbase[m];
+    totals[4 + 3 * m + 2] = 0;
+  }
+}
+
 // 168-byte SVONode records (SVO.h:20-35), one thread per 32-bit word.
 __global__ void k_pack_aos(NodeArrays n, uint64_t first, uint64_t count, uint32_t* dst) {
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -636,6 +719,17 @@ void launch_attrs_write(int semantics, const NodeArrays& nodes, uint64_t n_nodes
   uint64_t* mesh_first = (uint64_t*)out.totals + 4;  // [3 * n_meshes] after the totals
   hipLaunchKernelGGL(k_attr_write, dim3(blocks), dim3(BLOCK), 0, s, semantics, nodes, n_nodes,
                      mesh_node_base, n_meshes, blocks, (const uint64_t*)partial, out, mesh_first);
+}
+
+void launch_ref_attrs(const NodeArrays& nodes, uint64_t n_nodes, const uint64_t* mesh_node_base,
+                      uint32_t n_meshes, AttrOut out, hipStream_t s) {
+  hipLaunchKernelGGL(k_ref_totals, dim3((n_meshes + 255) / 256), dim3(BLOCK), 0, s, n_nodes,
+                     mesh_node_base, n_meshes, out.totals);
+  if (n_nodes == 0) return;
+  hipLaunchKernelGGL(k_ref_records, dim3((uint32_t)((n_nodes + REF_NODES - 1) / REF_NODES)),
+                     dim3(BLOCK), 0, s, (const float*)nodes.origin, (const float*)nodes.width,
+                     n_nodes, mesh_node_base, n_meshes, (uint32_t*)out.verts,
+                     (uint32_t*)out.edges);
 }
 
 void launch_pack_aos(const NodeArrays& nodes, uint64_t first, uint64_t n, void* dst,

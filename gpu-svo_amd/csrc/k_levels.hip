@@ -36,6 +36,37 @@ __device__ inline void block_range(uint64_t n, uint32_t G, uint32_t b, uint64_t&
   if (end > n) end = n;
 }
 
+// Loads keys [tb - 1, tb + tn] into s[0 .. tn + 1] with coalesced reads, so
+// that every element of the tile sees its neighbours in LDS.
+template <typename K>
+__device__ inline void load_tile(const K* src, uint64_t n, uint64_t tb, uint32_t tn, K* s) {
+  __syncthreads();  // the previous tile is no longer read
+  for (uint32_t j = threadIdx.x; j < tn + 2; j += BLOCK) {
+    const uint64_t gi = tb + j;  // element gi - 1
+    s[j] = (gi >= 1 && gi - 1 < n) ? src[gi - 1] : (K)0;
+  }
+  __syncthreads();
+}
+
+template <typename K>
+__device__ inline uint32_t first_flags_v(K prev, K cur, bool has_prev, KeyLayout<K> kl) {
+  if (!has_prev) return 3u;
+  if (cur == prev) return 0u;
+  return 1u | (kl.parent(cur) != kl.parent(prev) ? 2u : 0u);
+}
+
+template <typename K>
+__device__ inline uint32_t level_flags_v(K prev, K cur, K next, bool has_prev, bool has_next,
+                                         KeyLayout<K> kl, const MeshTreeState* state) {
+  const uint32_t m = kl.mesh(cur);
+  if (state[m].depth != 0) return 0u;
+  const bool first = !has_prev || kl.mesh(prev) != m;
+  const bool last = !has_next || kl.mesh(next) != m;
+  if (first && last) return 4u;
+  if (first) return 2u;
+  return kl.parent(cur) != kl.parent(prev) ? 2u : 0u;
+}
+
 // ---------------------------------------------------------------- level 1
 // flags of sorted hit key i: bit0 = first copy of its key, bit1 = parent-group head
 template <typename K>
@@ -109,18 +140,23 @@ __global__ __launch_bounds__(BLOCK) void k_first_write(const K* sorted,
                                                        uint32_t* first_child2, LevelSpan* spans,
                                                        uint32_t n_levels) {
   __shared__ uint64_t tmp[4];
+  __shared__ K sk[TILE + 2];
   const uint64_t n = *n_ptr;
   uint64_t beg, end;
   block_range(n, G, blockIdx.x, beg, end);
   uint64_t ubase = partial[2 * blockIdx.x], gbase = partial[2 * blockIdx.x + 1];
   for (uint64_t tb = beg; tb < end; tb += TILE) {
+    const uint32_t tn = (uint32_t)((end - tb) < TILE ? (end - tb) : TILE);
+    load_tile(sorted, n, tb, tn, sk);
     // blocked: thread t owns elements tb + t*EPT .. +EPT
-    const uint64_t i0 = tb + (uint64_t)threadIdx.x * EPT;
+    const uint32_t l0 = threadIdx.x * EPT;
+    const uint64_t i0 = tb + l0;
     uint32_t f[EPT];
     uint64_t nu = 0, ng = 0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      f[e] = (i0 + e < end) ? first_flags(sorted, i0 + e, kl) : 0u;
+      const uint32_t l = l0 + e;
+      f[e] = (l < tn) ? first_flags_v(sk[l], sk[l + 1], tb + l > 0, kl) : 0u;
       nu += f[e] & 1u;
       ng += f[e] >> 1;
     }
@@ -131,7 +167,7 @@ __global__ __launch_bounds__(BLOCK) void k_first_write(const K* sorted,
     for (int e = 0; e < EPT; ++e) {
       const uint64_t i = i0 + e;
       if (i >= end) break;
-      const K key = sorted[i];
+      const K key = sk[l0 + e + 1];
       const uint32_t m = kl.mesh(key);
       if (f[e] & 1u) {
         if (f[e] & 2u) {
@@ -142,14 +178,14 @@ __global__ __launch_bounds__(BLOCK) void k_first_write(const K* sorted,
         elem1[ru] = key;
         group1[ru] = (uint32_t)(rg - 1);
         // first element of its mesh
-        if (i == 0 || kl.mesh(sorted[i - 1]) != m) {
+        if (i == 0 || kl.mesh(sk[l0 + e]) != m) {
           spans[(uint64_t)m * n_levels].elem_first = (uint32_t)ru;
           spans[(uint64_t)m * n_levels].group_first = (uint32_t)(rg - 1);
         }
         ++ru;
       }
       // last element (copy) of its mesh
-      if (i + 1 == n || kl.mesh(sorted[i + 1]) != m) {
+      if (i + 1 == n || kl.mesh(sk[l0 + e + 2]) != m) {
         spans[(uint64_t)m * n_levels].elem_end = (uint32_t)ru;
         spans[(uint64_t)m * n_levels].group_end = (uint32_t)rg;
       }
@@ -211,18 +247,25 @@ __global__ __launch_bounds__(BLOCK) void k_level_write(
     MeshTreeState* state, uint32_t G, const uint64_t* partial, uint32_t* group_l, K* elem_next,
     uint32_t* first_child_next, LevelSpan* spans, uint32_t n_levels) {
   __shared__ uint64_t tmp[4];
+  __shared__ K sk[TILE + 2];
   const uint64_t n = *n_ptr;
   uint64_t beg, end;
   block_range(n, G, blockIdx.x, beg, end);
   uint64_t gbase = partial[2 * blockIdx.x + 1];
   const uint32_t li = (uint32_t)(level - 1);
   for (uint64_t tb = beg; tb < end; tb += TILE) {
-    const uint64_t i0 = tb + (uint64_t)threadIdx.x * EPT;
+    const uint32_t tn = (uint32_t)((end - tb) < TILE ? (end - tb) : TILE);
+    load_tile(elem, n, tb, tn, sk);
+    const uint32_t l0 = threadIdx.x * EPT;
+    const uint64_t i0 = tb + l0;
     uint32_t f[EPT];
     uint64_t ng = 0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      f[e] = (i0 + e < end) ? level_flags(elem, i0 + e, n, kl, state) : 0u;
+      const uint32_t l = l0 + e;
+      f[e] = (l < tn) ? level_flags_v(sk[l], sk[l + 1], sk[l + 2], tb + l > 0, tb + l + 1 < n,
+                                      kl, state)
+                      : 0u;
       ng += (f[e] >> 1) & 1u;
     }
     uint64_t tg;
@@ -231,7 +274,7 @@ __global__ __launch_bounds__(BLOCK) void k_level_write(
     for (int e = 0; e < EPT; ++e) {
       const uint64_t i = i0 + e;
       if (i >= end) break;
-      const K key = elem[i];
+      const K key = sk[l0 + e + 1];
       const uint32_t m = kl.mesh(key);
       if (f[e] & 4u) {  // the only key of its mesh at this level: the root
         state[m].depth = (uint32_t)level;
@@ -253,8 +296,8 @@ __global__ __launch_bounds__(BLOCK) void k_level_write(
         ++rg;
       }
       group_l[i] = (uint32_t)(rg - 1);
-      const bool first = (i == 0) || kl.mesh(elem[i - 1]) != m;
-      const bool last = (i + 1 == n) || kl.mesh(elem[i + 1]) != m;
+      const bool first = (i == 0) || kl.mesh(sk[l0 + e]) != m;
+      const bool last = (i + 1 == n) || kl.mesh(sk[l0 + e + 2]) != m;
       LevelSpan& sp = spans[(uint64_t)m * n_levels + li];
       if (first) {
         sp.elem_first = (uint32_t)i;

@@ -12,9 +12,10 @@
 //                 lanes walk their box row by row, and the yz-projection
 //                 test, which does not depend on x, is evaluated once per
 //                 row (an exact hoist: the same operands, the same rounding).
-//                 Larger triangles are cut into row chunks and queued.
-//  * k_vox_big    one wave64 per chunk: the row terms are wave-uniform and
-//                 the 64 lanes cover consecutive x.
+//                 Larger triangles are cut into chunks of 256 rows and queued.
+//  * k_vox_big    one wave64 per chunk: each lane prunes one row to the
+//                 x-range outside of which a test provably fails (row_range)
+//                 and tests the cells left in it.
 // Hits are staged per wave in LDS and flushed with one atomic per ~1K keys.
 #include "svo_internal.h"
 
@@ -261,6 +262,49 @@ __device__ inline bool cell_hit(const TriSetup& S, const RowTerms& R, int x) {
   return ok;
 }
 
+// Conservative x-range [xa, xb] of row (y, z): outside it the plane test or
+// one of the xy / zx edge tests fails for certain.  Each test is affine in x;
+// its root is computed in double and widened by two cells plus a bound on
+// the float rounding of the test (2^-20 of the magnitudes involved, several
+// times the few half-ulp roundings of the float evaluation).  Tests whose
+// x-slope is tiny are not used.  Skipping cells outside the range therefore
+// never changes the set of hits.
+__device__ inline void row_range(const TriSetup& S, const RowTerms& R, int& xa, int& xb) {
+  double lo = S.g0[0], hi = S.g1[0];
+  const double ux = S.u[0];
+  const double xmax = (double)S.g1[0] + 1.0;
+  {
+    const double a = (double)S.n[0] * ux;
+    const double d1 = S.d1, d2 = S.d2, A = R.A;
+    const double mag = fabs(a) * xmax + fabs(A) + fabs(d1) + fabs(d2);
+    if (fabs(a) > 1e-12 * (mag + 1e-30) && fabs(a) > 1e-15) {
+      const double err = mag * 0x1p-20 / fabs(a) + 2.0;
+      double x1 = (-fmax(d1, d2) - A) / a, x2 = (-fmin(d1, d2) - A) / a;
+      if (x1 > x2) {
+        const double t = x1;
+        x1 = x2;
+        x2 = t;
+      }
+      lo = fmax(lo, floor(x1 - err));
+      hi = fmin(hi, ceil(x2 + err));
+    }
+  }
+  for (int i = 0; i < 6; ++i) {
+    // xy: ne.x * px + rxy + de >= 0;  zx: rzx + ne.y * px + de >= 0
+    const double b = (double)(i < 3 ? S.ne[0][i][0] : S.ne[2][i - 3][1]) * ux;
+    const double c = i < 3 ? (double)R.rxy[i] + (double)S.de[0][i]
+                           : (double)R.rzx[i - 3] + (double)S.de[2][i - 3];
+    const double mag = fabs(b) * xmax + fabs(c) + fabs(i < 3 ? R.rxy[i] : R.rzx[i - 3]);
+    if (!(fabs(b) > 1e-12 * (mag + 1e-30) && fabs(b) > 1e-30)) continue;
+    const double err = mag * 0x1p-20 / fabs(b) + 2.0;
+    const double root = -c / b;
+    if (b > 0) lo = fmax(lo, floor(root - err));
+    else hi = fmin(hi, ceil(root + err));
+  }
+  xa = lo > (double)S.g1[0] ? S.g1[0] + 1 : (int)lo;
+  xb = hi < (double)S.g0[0] ? S.g0[0] - 1 : (int)hi;
+}
+
 template <typename K>
 struct Stager {
   K* buf;            // this wave's LDS stage
@@ -288,10 +332,8 @@ struct Stager {
   }
 };
 
-__device__ inline uint32_t rows_per_chunk(uint32_t chunk_cells, int nx) {
-  const uint32_t r = chunk_cells / (uint32_t)nx;
-  return r ? r : 1u;
-}
+// Big-triangle work item: 256 rows (y, z) of the box, four per lane.
+constexpr uint64_t BIG_ROWS = 256;
 
 __global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
                                                     const MeshParams* params,
@@ -311,10 +353,7 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
       tri_box(v, P, g0, g1);
       const uint64_t nx = g1[0] - g0[0] + 1, ny = g1[1] - g0[1] + 1, nz = g1[2] - g0[2] + 1;
       cells = nx * ny * nz;
-      if (cells > small_max) {
-        const uint64_t rpc = rows_per_chunk(chunk_cells, (int)nx);
-        chunks = (ny * nz + rpc - 1) / rpc;
-      }
+      if (cells > small_max) chunks = (ny * nz + BIG_ROWS - 1) / BIG_ROWS;
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -372,11 +411,10 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
       if (cells <= small_max) {
         count = cells;
       } else {
-        const uint64_t rpc = rows_per_chunk(chunk_cells, (int)nx);
-        const uint64_t nch = (ny * nz + rpc - 1) / rpc;
+        const uint64_t nch = (ny * nz + BIG_ROWS - 1) / BIG_ROWS;
         const uint64_t at = atomicAdd(big_count, (unsigned long long)nch);
         for (uint64_t j = 0; j < nch; ++j)
-          if (at + j < big_cap) big[at + j] = BigItem{(uint32_t)gid, (uint32_t)(j * rpc)};
+          if (at + j < big_cap) big[at + j] = BigItem{(uint32_t)gid, (uint32_t)(j * BIG_ROWS)};
       }
       mesh_key = (K)m << cb;
     }
@@ -411,6 +449,9 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
   st.finish();
 }
 
+// One wave per work item of 256 rows: each lane prunes four rows of the item
+// to their x-ranges and walks them; each inner loop runs as long as the
+// longest pruned row of the 64.
 template <typename K>
 __global__ __launch_bounds__(BLOCK) void k_vox_big(const MeshDev* meshes,
                                                    const MeshParams* params,
@@ -420,12 +461,15 @@ __global__ __launch_bounds__(BLOCK) void k_vox_big(const MeshDev* meshes,
                                                    const unsigned long long* big_count,
                                                    uint64_t n_tris, K* keys,
                                                    unsigned long long* key_count) {
+  (void)chunk_cells;
   __shared__ K stage[WAVES][STAGE];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t lane = __lane_id();
   Stager<K> st;
-  st.buf = stage[threadIdx.x >> 6];
+  st.buf = stage[w];
   st.keys = keys;
   st.count = key_count;
-  const uint64_t item_id = (uint64_t)blockIdx.x * WAVES + (threadIdx.x >> 6);
+  const uint64_t item_id = (uint64_t)blockIdx.x * WAVES + w;
   const uint64_t queued = *big_count;
   const uint64_t n_items = queued < big_cap ? queued : big_cap;
   if (item_id < n_items && big[item_id].tri < n_tris) {
@@ -437,20 +481,19 @@ __global__ __launch_bounds__(BLOCK) void k_vox_big(const MeshDev* meshes,
     load_tri(md, P, it.tri - md.tri_base, v);  // indices were checked when queued
     TriSetup S;
     tri_setup(v, P, S);
-    const int nx = S.g1[0] - S.g0[0] + 1, ny = S.g1[1] - S.g0[1] + 1,
-              nz = S.g1[2] - S.g0[2] + 1;
-    const uint64_t rows = (uint64_t)ny * nz;
-    const uint64_t rpc = rows_per_chunk(chunk_cells, nx);
-    const uint64_t r_end = it.row0 + rpc < rows ? it.row0 + rpc : rows;
+    const uint64_t ny = S.g1[1] - S.g0[1] + 1, nz = S.g1[2] - S.g0[2] + 1;
     const K mesh_key = (K)m << cb;
-    for (uint64_t r = it.row0; r < r_end; ++r) {
-      const int y = S.g0[1] + (int)(r % (uint64_t)ny);
-      const int z = S.g0[2] + (int)(r / (uint64_t)ny);
+    for (uint64_t k = 0; k < BIG_ROWS; k += 64) {
+      const uint64_t r = (uint64_t)it.row0 + k + lane;
+      int y = 0, z = 0, xa = 0, xb = -1;
       RowTerms R;
-      if (!row_terms(S, y, z, R)) continue;
-      for (int xb = S.g0[0]; xb <= S.g1[0]; xb += 64) {
-        const int x = xb + (int)__lane_id();
-        const bool hit = x <= S.g1[0] && cell_hit(S, R, x);
+      if (r < ny * nz) {
+        y = S.g0[1] + (int)(r % ny);
+        z = S.g0[2] + (int)(r / ny);
+        if (row_terms(S, y, z, R)) row_range(S, R, xa, xb);
+      }
+      for (int x = xa; __any(x <= xb); ++x) {
+        const bool hit = x <= xb && cell_hit(S, R, x);
         st.push(hit, mesh_key | morton3<K>((uint32_t)x, (uint32_t)y, (uint32_t)z));
       }
     }

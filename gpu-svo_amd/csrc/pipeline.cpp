@@ -18,7 +18,9 @@
 // and constructNodeAtrributes (SVO.cu:730-735).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
+#include <map>
 
 #include "svo_host.h"
 
@@ -45,6 +47,10 @@ int Context::init(int dev, uint64_t bud) {
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || dev < 0 || dev >= n) return SVO_E_HIP;
   HIP_TRY(hipSetDevice(dev));
   HIP_TRY(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
+  // freed blocks stay cached up to 80% of the device (hipMalloc of a large
+  // block maps its pages and costs seconds at 100+ GB)
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b) cache_limit = total_b / 10 * 8;
   return SVO_OK;
 }
 
@@ -59,6 +65,7 @@ void Context::destroy() {
   }
   lanes.clear();
   free_lanes.clear();
+  trim_cache(0);
   if (aux_stream) hipStreamDestroy(aux_stream);
   aux_stream = nullptr;
 }
@@ -96,25 +103,74 @@ int Context::reserve(uint64_t bytes) {
 }
 void Context::release(uint64_t bytes) { in_use.fetch_sub(bytes); }
 
+// Size classes 2^k * {1, 1.25, 1.5, 1.75} above 1 MiB (at most 25% slack),
+// 256-byte granules below.
+size_t Context::size_class(size_t bytes) {
+  if (bytes == 0) bytes = 256;
+  if (bytes <= (1u << 20)) return (bytes + 255) & ~(size_t)255;
+  int k = 63 - __builtin_clzll((unsigned long long)bytes);
+  const size_t base = (size_t)1 << k;
+  const size_t step = base >> 2;
+  return base + ((bytes - base + step - 1) / step) * step;
+}
+
 int Context::dev_malloc(void** p, size_t bytes) {
   *p = nullptr;
-  if (bytes == 0) bytes = 256;
-  SVO_TRY(reserve(bytes));
+  const size_t cls = size_class(bytes);
+  SVO_TRY(reserve(cls));
+  {
+    std::lock_guard<std::mutex> lk(cache_mu);
+    auto it = cache.find(cls);
+    if (it != cache.end()) {
+      *p = it->second;
+      cache.erase(it);
+      cached_bytes -= cls;
+      return SVO_OK;
+    }
+  }
   hipSetDevice(device);
-  if (hipMalloc(p, bytes) != hipSuccess) {
+  if (hipMalloc(p, cls) != hipSuccess) {
     hipGetLastError();
-    release(bytes);
-    *p = nullptr;
-    return SVO_E_OOM;
+    trim_cache(0);  // give cached blocks back and retry once
+    if (hipMalloc(p, cls) != hipSuccess) {
+      hipGetLastError();
+      release(cls);
+      *p = nullptr;
+      return SVO_E_OOM;
+    }
   }
   return SVO_OK;
 }
 
 void Context::dev_free(void* p, size_t bytes) {
   if (!p) return;
+  const size_t cls = size_class(bytes);
+  release(cls);
+  {
+    std::lock_guard<std::mutex> lk(cache_mu);
+    cache.emplace(cls, p);
+    cached_bytes += cls;
+  }
+  uint64_t limit = cache_limit;
+  if (budget) limit = std::min<uint64_t>(limit, budget / 2);
+  if (cached_bytes > limit) trim_cache(limit);
+}
+
+void Context::trim_cache(uint64_t keep_bytes) {
+  std::vector<void*> drop;
+  {
+    std::lock_guard<std::mutex> lk(cache_mu);
+    // release the largest blocks first
+    while (cached_bytes > keep_bytes && !cache.empty()) {
+      auto it = std::prev(cache.end());
+      cached_bytes -= it->first;
+      drop.push_back(it->second);
+      cache.erase(it);
+    }
+  }
+  if (drop.empty()) return;
   hipSetDevice(device);
-  hipFree(p);
-  release(bytes ? bytes : 256);
+  for (void* q : drop) hipFree(q);
 }
 
 DeviceBlock::~DeviceBlock() {
@@ -250,6 +306,34 @@ struct StageTimer {
 // ------------------------------------------------------------------ the pass
 namespace {
 
+// True when the reference's consecutive unique (uniqueVert / uniqueEdge,
+// SVO.cu:679-720) provably removes nothing: every node width w is positive,
+// finite and at least 2^-22 times the largest coordinate magnitude a node
+// corner can have, so o + w != o on every axis.  Then the eight corners
+// w*(bits) of a node are pairwise distinct, corner 7 of a node differs from
+// corner 0 of the next, all eight absolute corners of a node are distinct,
+// adjacent edges of a node differ, and edge 11 of a node (along z) cannot
+// equal edge 0 of the next (along y).
+bool no_adjacent_duplicates(const std::vector<MeshNodeLayout>& lay, const MeshParams* par,
+                            const std::vector<int>& mstat) {
+  for (size_t i = 0; i < lay.size(); ++i) {
+    if (mstat[i] != SVO_OK || lay[i].n_nodes == 0) continue;
+    for (uint32_t l = 0; l < lay[i].depth; ++l) {
+      const double w = lay[i].width[l];
+      if (!(w > 0.0) || !std::isfinite(w)) return false;
+      // node origins are gridOrigin + w * coord with coord < ceil(res / 2^(l-1))
+      // on every axis (the width is unitVoxelSize.x on all axes, SVO.cu:351)
+      double M = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        const double cells = std::ceil((double)par[i].res[a] / std::ldexp(1.0, (int)l));
+        M = std::max(M, std::fabs((double)par[i].bmin[a]) + w * (cells + 2.0));
+      }
+      if (w < M * 0x1p-22) return false;
+    }
+  }
+  return true;
+}
+
 struct Plan {
   std::vector<uint32_t> idx;  // active mesh -> input index
   int bits = 1;               // per-axis coordinate bits of the batch
@@ -321,15 +405,14 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   MeshParams* d_par = (MeshParams*)(ctl + sz_md + sz_tb + sz_acc);
   unsigned long long* d_cnt = (unsigned long long*)(ctl + sz_md + sz_tb + sz_acc + sz_par);
   // d_cnt: [0] box cells, [1] big chunks, [2] keys emitted, [3] big queued
-  const size_t pin_small = 64 + sizeof(MeshParams) * n;
-  SVO_TRY(L.ensure_pinned(pin_small));
+  SVO_TRY(L.ensure_pinned(sz_md + sz_tb + 64 + sizeof(MeshParams) * n));
   {
-    // stage host->device control data through the pinned buffer
+    // stage the mesh descriptors and triangle bases through the pinned
+    // buffer in one copy; the host does not touch the buffer again before
+    // the next stream synchronization
     std::memcpy(L.pinned, md.data(), sizeof(MeshDev) * n);
-    HIP_TRY(hipMemcpyAsync(d_md, L.pinned, sizeof(MeshDev) * n, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    std::memcpy(L.pinned, tri_base.data(), 8 * (n + 1));
-    HIP_TRY(hipMemcpyAsync(d_tb, L.pinned, 8 * (n + 1), hipMemcpyHostToDevice, s));
+    std::memcpy((char*)L.pinned + sz_md, tri_base.data(), 8 * (n + 1));
+    HIP_TRY(hipMemcpyAsync(d_md, L.pinned, sz_md + sz_tb, hipMemcpyHostToDevice, s));
   }
   launch_fill_u32((uint32_t*)d_cnt, 0u, 16, s);
   timer.mark("start");
@@ -566,12 +649,17 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   timer.mark("neighbors");
   void* attr_tmp = nullptr;
   if (want_attrs && N) {
-    SVO_TRY(W.get("attr_tmp", attr_tmp_bytes(C.tuning.scan_blocks), &attr_tmp));
     AttrOut ao{out->verts, out->edges, out->faces, d_tot};
-    const uint32_t ab = grid_for(N, 2048, C.tuning.scan_blocks);
-    launch_attrs_count(sem, na, N, d_nbase, n, ab, attr_tmp, ao, s);
-    timer.mark("attr_count");
-    launch_attrs_write(sem, na, N, d_nbase, n, ab, attr_tmp, ao, s);
+    if (sem == SVO_SEM_REFERENCE && no_adjacent_duplicates(lay, h_par, mstat)) {
+      timer.mark("attr_count");
+      launch_ref_attrs(na, N, d_nbase, n, ao, s);
+    } else {
+      SVO_TRY(W.get("attr_tmp", attr_tmp_bytes(C.tuning.scan_blocks), &attr_tmp));
+      const uint32_t ab = grid_for(N, 2048, C.tuning.scan_blocks);
+      launch_attrs_count(sem, na, N, d_nbase, n, ab, attr_tmp, ao, s);
+      timer.mark("attr_count");
+      launch_attrs_write(sem, na, N, d_nbase, n, ab, attr_tmp, ao, s);
+    }
   }
   timer.mark("attr_write");
   uint64_t* h_tot = (uint64_t*)L.pinned;

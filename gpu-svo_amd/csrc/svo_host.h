@@ -3,6 +3,7 @@
 #pragma once
 #include <atomic>
 #include <condition_variable>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -79,8 +80,16 @@ struct Context {
   void release_lane(Lane* l);
   int reserve(uint64_t bytes);  // budget accounting; SVO_E_OOM if over
   void release(uint64_t bytes);
+  // Device allocations go through a size-class cache: freed blocks are kept
+  // for reuse (hipFree synchronizes the device) up to cache_limit bytes.
   int dev_malloc(void** p, size_t bytes);
   void dev_free(void* p, size_t bytes);
+  void trim_cache(uint64_t keep_bytes);
+  static size_t size_class(size_t bytes);
+  std::mutex cache_mu;
+  std::multimap<size_t, void*> cache;
+  uint64_t cached_bytes = 0;
+  uint64_t cache_limit = 96ull << 30;
 };
 
 }  // namespace svo

@@ -129,6 +129,10 @@ void launch_attrs_count(int semantics, const NodeArrays& nodes, uint64_t n_nodes
 void launch_attrs_write(int semantics, const NodeArrays& nodes, uint64_t n_nodes,
                         const uint64_t* mesh_node_base, uint32_t n_meshes, uint32_t blocks,
                         void* tmp, AttrOut out, hipStream_t s);
+// Reference-semantics records at identity offsets (8 and 12 per node), valid
+// when no record can equal its predecessor; also writes the totals.
+void launch_ref_attrs(const NodeArrays& nodes, uint64_t n_nodes, const uint64_t* mesh_node_base,
+                      uint32_t n_meshes, AttrOut out, hipStream_t s);
 // 168-byte SVONode records for nodes [first, first + n) (32-bit codes).
 void launch_pack_aos(const NodeArrays& nodes, uint64_t first, uint64_t n, void* dst,
                      hipStream_t s);
