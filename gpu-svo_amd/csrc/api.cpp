@@ -325,6 +325,35 @@ int svo_write_obj(const svo_tree* tree, const char* dir, const char* stem, char*
   return write_tree_obj(path.c_str(), org.data(), w.data(), N);
 }
 
+int svo_write_voxels_obj(const svo_tree* tree, const char* dir, const char* stem,
+                         char* path_out, size_t path_cap) {
+  if (!tree || !stem) return SVO_E_INVALID;
+  // base_filename + "_" + treeDepth + "_voxel.obj" with treeDepth == 1, SVO.cu:412, 757-758
+  std::string path = (dir && *dir) ? std::string(dir) + "/" : std::string();
+  path += std::string(stem) + "_1_voxel.obj";
+  if (path_out && path_cap) {
+    std::strncpy(path_out, path.c_str(), path_cap - 1);
+    path_out[path_cap - 1] = 0;
+  }
+  const uint64_t U = tree->info.n_voxels;
+  const int cb = tree->key_bits_out / 8;
+  std::vector<uint8_t> raw(cb * U);
+  int st = svo_tree_copy_field(tree, SVO_FIELD_VOXELS, raw.data(), raw.size());
+  if (st != SVO_OK) return st;
+  // modelBBox.min + width * Vector3f(x, y, z), width = unitNodeWidth, SVO.cu:769-770
+  const float w = tree->info.unit[0];
+  std::vector<float> org(3 * U), wid(U, w);
+  for (uint64_t i = 0; i < U; ++i) {
+    uint64_t code = 0;
+    std::memcpy(&code, raw.data() + cb * i, cb);
+    uint32_t c[3] = {0, 0, 0};
+    for (int b = 0; b < 21; ++b)
+      for (int a = 0; a < 3; ++a) c[a] |= (uint32_t)((code >> (3 * b + a)) & 1u) << b;
+    for (int a = 0; a < 3; ++a) org[3 * i + a] = tree->info.bbox_min[a] + w * (float)c[a];
+  }
+  return write_tree_obj(path.c_str(), org.data(), wid.data(), U);
+}
+
 int svo_query_neighbors(const svo_tree* tree, const uint32_t* node, const uint8_t* slot,
                         uint64_t n, uint32_t* out) {
   if (!tree || (n && (!node || !slot || !out))) return SVO_E_INVALID;

@@ -1,19 +1,27 @@
-// k_sort.hip -- LSD radix sort of voxel keys, 8-bit digits, wave64 ranking.
+// k_sort.hip -- LSD radix sort of voxel keys: 8-bit digits, one read for all
+// digit histograms, then one kernel per digit pass (onesweep).
 //
 // Replaces the reference's dense flag array + exclusive scan + compaction
 // (SVO.cu:367-405), which is O(res^3), with a sort of the emitted hit keys,
 // O(hits).  The sorted order equals the reference's index order of the dense
 // array (keys are the Morton codes).
 //
-// Fixed grid of G blocks; block b owns the contiguous key range
-// [b*ceil(n/G), (b+1)*ceil(n/G)) where n is read from device memory, so the
-// launch shapes do not depend on the hit count.  Per digit pass:
-//   k_count    per-block digit histogram (per-wave LDS histograms)
-//   k_rowscan  one block per digit: exclusive scan over the G blocks
-//   k_scatter  per 4096-key tile: stable ranking with wave64 match masks
-//              (8 ballots per 64 keys), LDS reorder by digit, then writes
-//              that are contiguous within each digit run.
-// Stable, so the passes compose into a full LSD sort.
+// The key count n is read from device memory, so no launch shape depends on
+// the hit count.  Kernels:
+//   k_sort_clear  zero the histograms, tile tickets and the status words of
+//                 the tiles the n keys occupy
+//   k_upsweep     digit histograms of every pass in one read of the keys
+//   k_scan_hist   exclusive scan of each pass's histogram -> digit offsets
+//   k_onesweep    per pass: persistent blocks claim 4096-key tiles by atomic
+//                 ticket, rank the keys stably with wave64 match masks
+//                 (8 ballots per 64 keys), publish per-digit tile counts and
+//                 find their exclusive prefix over earlier tiles by
+//                 decoupled look-back, then write through an LDS reorder so
+//                 the stores are contiguous within each digit run.
+// Stable, so the passes compose into a full LSD sort.  HBM traffic for n keys
+// of B bytes and P passes: n*B (upsweep) + P * 2*n*B.
+#include <algorithm>
+
 #include "svo_internal.h"
 #include "svo_scan.h"
 
@@ -23,83 +31,146 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / 64;
 constexpr int RADIX = 256;
+constexpr int MAX_PASSES = 8;
 constexpr int KPT = 16;                       // keys per lane per tile
 constexpr int TILE = WAVES * 64 * KPT;        // 4096
+constexpr int UP_KEYS = 16;                   // consecutive keys per upsweep thread
+constexpr int LOOKBACK = 16;                  // status words loaded per look-back step
 
-__device__ inline void block_range(uint64_t n, uint32_t G, uint32_t b, uint64_t& beg,
-                                   uint64_t& end) {
-  const uint64_t chunk = (n + G - 1) / G;
-  beg = (uint64_t)b * chunk;
-  if (beg > n) beg = n;
-  end = beg + chunk;
-  if (end > n) end = n;
+// Status word of (tile, digit): bits 62-63 flag (1 = tile count only,
+// 2 = inclusive prefix), bits 32-61 pass epoch (pass index + 1), bits 0-31
+// value.  The words of the used tiles are zeroed before the first pass, so a
+// word whose epoch is not the current pass's has not been written yet.
+constexpr uint64_t ST_AGG = 1ull << 62, ST_INCL = 2ull << 62;
+constexpr uint64_t ST_FLAGS = 3ull << 62;
+
+// Digit of the pass at `shift` covers bits [shift, min(shift + 8, bits)).
+__host__ __device__ inline uint32_t digit_mask(int bits, int shift) {
+  const int w = bits - shift < 8 ? bits - shift : 8;
+  return (1u << w) - 1u;
+}
+
+__device__ inline uint64_t st_word(uint64_t flag, uint32_t epoch, uint32_t v) {
+  return flag | ((uint64_t)epoch << 32) | v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sort_clear(uint64_t* status, uint32_t* hist,
+                                                      uint32_t* tickets,
+                                                      const unsigned long long* n_ptr) {
+  const uint64_t tiles = (*n_ptr + TILE - 1) / TILE;
+  const uint64_t words = tiles * RADIX;
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < words; i += stride)
+    status[i] = 0;
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < MAX_PASSES * RADIX; i += BLOCK) hist[i] = 0;
+    if (threadIdx.x < MAX_PASSES) tickets[threadIdx.x] = 0;
+  }
 }
 
 template <typename K>
-__global__ __launch_bounds__(BLOCK) void k_count(const K* in, const unsigned long long* n_ptr,
-                                                 int shift, uint32_t G, uint32_t* counts) {
-  __shared__ uint32_t hist[WAVES][RADIX];
-  const uint32_t w = threadIdx.x >> 6;
-  for (int i = threadIdx.x; i < WAVES * RADIX; i += BLOCK) (&hist[0][0])[i] = 0;
+__device__ inline void load_run(const K* p, bool vec, uint32_t cnt, K (&kk)[UP_KEYS]) {
+  if (vec && cnt == UP_KEYS) {
+    const uint4* q = (const uint4*)p;
+    constexpr int PER = 16 / sizeof(K);
+#pragma unroll
+    for (int v = 0; v < UP_KEYS / PER; ++v) {
+      const uint4 x = q[v];
+      const K* e = (const K*)&x;
+#pragma unroll
+      for (int j = 0; j < PER; ++j) kk[v * PER + j] = e[j];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < UP_KEYS; ++j) kk[j] = (uint32_t)j < cnt ? p[j] : (K)0;
+  }
+}
+
+// Each thread owns UP_KEYS consecutive keys and adds runs of equal digits to
+// the block histogram: high digits (mesh index, coarse cells) come in long
+// runs because nearby triangles emit nearby keys, which keeps LDS atomics
+// off the hot bins.
+template <typename K>
+__global__ __launch_bounds__(BLOCK) void k_upsweep(const K* in, const unsigned long long* n_ptr,
+                                                   int bits, uint32_t* hist) {
+  const int passes = (bits + 7) / 8;
+  __shared__ uint32_t h[MAX_PASSES][RADIX];
+  for (int i = threadIdx.x; i < MAX_PASSES * RADIX; i += BLOCK) (&h[0][0])[i] = 0;
   __syncthreads();
-  uint64_t beg, end;
-  block_range(*n_ptr, G, blockIdx.x, beg, end);
-  for (uint64_t i = beg + threadIdx.x; i < end; i += BLOCK) {
-    const uint32_t d = (uint32_t)(in[i] >> shift) & (RADIX - 1);
-    atomicAdd(&hist[w][d], 1u);
+  const uint64_t n = *n_ptr;
+  const uint64_t chunks = (n + UP_KEYS - 1) / UP_KEYS;
+  const bool vec = ((uintptr_t)in & 15) == 0;
+  for (uint64_t c = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; c < chunks;
+       c += (uint64_t)gridDim.x * BLOCK) {
+    const uint64_t b = c * UP_KEYS;
+    const uint32_t cnt = (uint32_t)((n - b) < UP_KEYS ? (n - b) : UP_KEYS);
+    K kk[UP_KEYS];
+    load_run(in + b, vec, cnt, kk);
+    for (int p = 0; p < passes; ++p) {
+      const int sh = 8 * p;
+      const uint32_t dm = digit_mask(bits, sh);
+      uint32_t last = (uint32_t)(kk[0] >> sh) & dm, run = 1;
+#pragma unroll
+      for (int j = 1; j < UP_KEYS; ++j) {
+        if ((uint32_t)j < cnt) {
+          const uint32_t d = (uint32_t)(kk[j] >> sh) & dm;
+          if (d == last) {
+            ++run;
+          } else {
+            atomicAdd(&h[p][last], run);
+            last = d;
+            run = 1;
+          }
+        }
+      }
+      atomicAdd(&h[p][last], run);
+    }
   }
   __syncthreads();
-  const uint32_t d = threadIdx.x;
-  uint32_t s = 0;
-  for (int k = 0; k < WAVES; ++k) s += hist[k][d];
-  counts[(uint64_t)d * G + blockIdx.x] = s;
+  for (int i = threadIdx.x; i < passes * RADIX; i += BLOCK) {
+    const uint32_t v = (&h[0][0])[i];
+    if (v) atomicAdd(hist + i, v);
+  }
 }
 
-// Block d: exclusive scan of counts[d][0..G) in place, row total to row_tot[d].
-__global__ __launch_bounds__(BLOCK) void k_rowscan(uint32_t* counts, uint32_t G,
-                                                   uint32_t* row_tot) {
+__global__ __launch_bounds__(BLOCK) void k_scan_hist(const uint32_t* hist, int passes,
+                                                     uint32_t* offs) {
   __shared__ uint32_t tmp[4];
-  uint32_t* row = counts + (uint64_t)blockIdx.x * G;
-  const uint32_t per = (G + BLOCK - 1) / BLOCK;
-  const uint32_t b0 = threadIdx.x * per;
-  uint32_t local = 0;
-  for (uint32_t k = 0; k < per; ++k)
-    if (b0 + k < G) local += row[b0 + k];
-  uint32_t total;
-  uint32_t run = block_excl_scan256(local, tmp, total);
-  for (uint32_t k = 0; k < per; ++k)
-    if (b0 + k < G) {
-      const uint32_t c = row[b0 + k];
-      row[b0 + k] = run;
-      run += c;
-    }
-  if (threadIdx.x == 0) row_tot[blockIdx.x] = total;
+  for (int p = 0; p < passes; ++p) {
+    uint32_t total;
+    offs[p * RADIX + threadIdx.x] =
+        block_excl_scan256(hist[p * RADIX + threadIdx.x], tmp, total);
+  }
 }
 
+// One digit pass.  Blocks loop taking tickets; a ticket holder is always a
+// running block, so the look-back of tile t only waits on tiles < t whose
+// blocks are already past their ticket and publish without waiting on any
+// later tile: every wait ends.  Tickets at or past the last tile exit.
 template <typename K>
-__global__ __launch_bounds__(BLOCK) void k_scatter(const K* in, K* out,
-                                                   const unsigned long long* n_ptr, int shift,
-                                                   uint32_t G, const uint32_t* counts,
-                                                   const uint32_t* row_tot) {
+__global__ __launch_bounds__(BLOCK) void k_onesweep(const K* in, K* out,
+                                                    const unsigned long long* n_ptr, int shift,
+                                                    uint32_t dm, const uint32_t* offs,
+                                                    uint64_t* status,
+                                                    uint32_t* ticket, uint32_t epoch) {
   __shared__ uint32_t wcnt[WAVES][RADIX];
   __shared__ uint32_t dbase[RADIX];
   __shared__ uint32_t tstart[RADIX];
-  __shared__ uint32_t tcount[RADIX];
   __shared__ uint32_t tmp[4];
+  __shared__ uint32_t s_tile;
   __shared__ K sorted[TILE];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = __lane_id();
-  uint64_t beg, end;
-  block_range(*n_ptr, G, blockIdx.x, beg, end);
-  {
-    uint32_t total;
-    const uint32_t ex = block_excl_scan256(row_tot[threadIdx.x], tmp, total);
-    dbase[threadIdx.x] = ex + counts[(uint64_t)threadIdx.x * G + blockIdx.x];
-  }
-  for (uint64_t tb = beg; tb < end; tb += TILE) {
-    const uint32_t tn = (uint32_t)((end - tb) < TILE ? (end - tb) : TILE);
+  const uint64_t n = *n_ptr;
+  const uint32_t d_off = offs[threadIdx.x];
+  while (true) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
     for (int i = lane; i < RADIX; i += 64) wcnt[w][i] = 0;
     __syncthreads();
+    const uint64_t tile = s_tile;
+    const uint64_t tb = tile * TILE;
+    if (tb >= n) return;  // uniform over the block
+    const uint32_t tn = (uint32_t)((n - tb) < TILE ? (n - tb) : TILE);
     K keys[KPT];
     uint32_t rank[KPT];
 #pragma unroll
@@ -107,7 +178,7 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(const K* in, K* out,
       const uint32_t pos = w * (KPT * 64) + it * 64 + lane;
       const bool valid = pos < tn;
       const K key = valid ? in[tb + pos] : (K)0;
-      const uint32_t d = (uint32_t)(key >> shift) & (RADIX - 1);
+      const uint32_t d = (uint32_t)(key >> shift) & dm;
       uint64_t eq = __ballot(valid);
 #pragma unroll
       for (int bit = 0; bit < 8; ++bit) {
@@ -128,58 +199,115 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(const K* in, K* out,
     __syncthreads();
     {
       const uint32_t d = threadIdx.x;
-      uint32_t s = 0;
+      uint32_t cnt = 0;
 #pragma unroll
       for (int k = 0; k < WAVES; ++k) {
         const uint32_t c = wcnt[k][d];
-        wcnt[k][d] = s;
-        s += c;
+        wcnt[k][d] = cnt;
+        cnt += c;
       }
-      tcount[d] = s;
+      uint64_t* st = status + tile * RADIX + d;
+      uint32_t prefix = 0;
+      if (tile == 0) {
+        __hip_atomic_store(st, st_word(ST_INCL, epoch, cnt), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __hip_atomic_store(st, st_word(ST_AGG, epoch, cnt), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        // Look back over a window of LOOKBACK earlier tiles at a time: the
+        // loads are independent, so one memory latency covers the window.
+        // Words are consumed nearest first until an inclusive one; a word
+        // not yet published ends the window and is loaded again.
+        int64_t t = (int64_t)tile - 1;
+        while (true) {
+          uint64_t wv[LOOKBACK];
+#pragma unroll
+          for (int j = 0; j < LOOKBACK; ++j)
+            wv[j] = t - j >= 0 ? __hip_atomic_load(status + (uint64_t)(t - j) * RADIX + d,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : 0ull;
+          int used = 0;
+          bool stop = false, found = false;
+#pragma unroll
+          for (int j = 0; j < LOOKBACK; ++j) {
+            if (!stop) {
+              const uint64_t x = wv[j];
+              if ((x & ST_FLAGS) == 0 || (uint32_t)((x >> 32) & 0x3FFFFFFFu) != epoch) {
+                stop = true;
+              } else {
+                prefix += (uint32_t)x;
+                ++used;
+                if ((x & ST_FLAGS) == ST_INCL) stop = found = true;
+              }
+            }
+          }
+          if (found) break;
+          // tile 0 always publishes an inclusive word, so t - used stays >= 0
+          t -= used;
+          if (used < LOOKBACK) __builtin_amdgcn_s_sleep(1);
+        }
+        __hip_atomic_store(st, st_word(ST_INCL, epoch, prefix + cnt), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+      }
+      dbase[d] = d_off + prefix;
       uint32_t total;
-      tstart[d] = block_excl_scan256(s, tmp, total);
+      tstart[d] = block_excl_scan256(cnt, tmp, total);
     }
     __syncthreads();
 #pragma unroll
     for (int it = 0; it < KPT; ++it) {
       const uint32_t pos = w * (KPT * 64) + it * 64 + lane;
       if (pos < tn) {
-        const uint32_t d = (uint32_t)(keys[it] >> shift) & (RADIX - 1);
+        const uint32_t d = (uint32_t)(keys[it] >> shift) & dm;
         sorted[tstart[d] + wcnt[w][d] + rank[it]] = keys[it];
       }
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < tn; i += BLOCK) {
       const K key = sorted[i];
-      const uint32_t d = (uint32_t)(key >> shift) & (RADIX - 1);
+      const uint32_t d = (uint32_t)(key >> shift) & dm;
       out[(uint64_t)dbase[d] + (i - tstart[d])] = key;
     }
-    __syncthreads();
-    dbase[threadIdx.x] += tcount[threadIdx.x];
     __syncthreads();
   }
 }
 
 }  // namespace
 
-size_t radix_sort_tmp_bytes(uint32_t blocks) {
-  return (size_t)RADIX * blocks * sizeof(uint32_t) + RADIX * sizeof(uint32_t);
+size_t radix_sort_tmp_bytes(uint64_t cap) {
+  const uint64_t tiles = (std::max<uint64_t>(cap, 1) + TILE - 1) / TILE;
+  return (size_t)tiles * RADIX * sizeof(uint64_t) + 2 * MAX_PASSES * RADIX * sizeof(uint32_t) +
+         256;
 }
 
 template <typename K>
 void launch_radix_sort(K* a, K* b, const unsigned long long* n, uint64_t cap, int bits,
                        uint32_t blocks, void* tmp, K** result, hipStream_t s) {
-  (void)cap;
-  uint32_t* counts = (uint32_t*)tmp;
-  uint32_t* row_tot = counts + (size_t)RADIX * blocks;
+  const uint64_t tiles = (std::max<uint64_t>(cap, 1) + TILE - 1) / TILE;
+  uint64_t* status = (uint64_t*)tmp;
+  uint32_t* hist = (uint32_t*)(status + tiles * RADIX);
+  uint32_t* offs = hist + MAX_PASSES * RADIX;
+  uint32_t* tickets = offs + MAX_PASSES * RADIX;
+  const int passes = std::min((bits + 7) / 8, MAX_PASSES);
   K* src = a;
   K* dst = b;
-  for (int shift = 0; shift < bits; shift += 8) {
-    hipLaunchKernelGGL(k_count<K>, dim3(blocks), dim3(BLOCK), 0, s, (const K*)src, n, shift,
-                       blocks, counts);
-    hipLaunchKernelGGL(k_rowscan, dim3(RADIX), dim3(BLOCK), 0, s, counts, blocks, row_tot);
-    hipLaunchKernelGGL(k_scatter<K>, dim3(blocks), dim3(BLOCK), 0, s, (const K*)src, dst, n,
-                       shift, blocks, (const uint32_t*)counts, (const uint32_t*)row_tot);
+  if (passes > 0) {
+    const uint32_t cb = (uint32_t)std::min<uint64_t>(2048, (tiles * RADIX + BLOCK - 1) / BLOCK);
+    hipLaunchKernelGGL(k_sort_clear, dim3(std::max<uint32_t>(cb, 1)), dim3(BLOCK), 0, s, status,
+                       hist, tickets, n);
+    const uint64_t chunks = (cap + UP_KEYS - 1) / UP_KEYS;
+    const uint32_t ub =
+        (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(1, (chunks + BLOCK - 1) / BLOCK));
+    hipLaunchKernelGGL(k_upsweep<K>, dim3(ub), dim3(BLOCK), 0, s, (const K*)a, n, bits, hist);
+    hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(BLOCK), 0, s, (const uint32_t*)hist, passes,
+                       offs);
+  }
+  const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, blocks));
+  for (int p = 0; p < passes; ++p) {
+    hipLaunchKernelGGL(k_onesweep<K>, dim3(grid), dim3(BLOCK), 0, s, (const K*)src, dst, n,
+                       8 * p, digit_mask(bits, 8 * p), (const uint32_t*)(offs + p * RADIX),
+                       status, tickets + p,
+                       (uint32_t)(p + 1));
     K* t = src;
     src = dst;
     dst = t;

@@ -13,9 +13,10 @@
 //                 test, which does not depend on x, is evaluated once per
 //                 row (an exact hoist: the same operands, the same rounding).
 //                 Larger triangles are cut into chunks of 256 rows and queued.
-//  * k_vox_big    one wave64 per chunk: each lane prunes one row to the
-//                 x-range outside of which a test provably fails (row_range)
-//                 and tests the cells left in it.
+//  * k_vox_big    one wave64 per queued item of 256 rows: each lane takes
+//                 four rows, prunes each to the x-range outside of which a
+//                 test provably fails (row_range) and tests the cells left
+//                 in it.
 // Hits are staged per wave in LDS and flushed with one atomic per ~1K keys.
 #include "svo_internal.h"
 

@@ -23,11 +23,6 @@ struct DevBufs {
   }
 };
 
-uint32_t blocks_for(uint64_t n, uint64_t per, uint32_t max_blocks) {
-  const uint64_t b = (n + per - 1) / per;
-  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, max_blocks));
-}
-
 }  // namespace
 
 extern "C" {
@@ -39,10 +34,10 @@ int svo_k_radix_sort(int device, void* keys, uint64_t n, int key_bytes, int bits
   if (n == 0) return SVO_OK;
   if (hipSetDevice(device) != hipSuccess) return SVO_E_HIP;
   DevBufs B;
-  const uint32_t blocks = blocks_for(n, 16384, 1024);
+  const uint32_t blocks = Tuning{}.sort_blocks;
   void* a = B.get(key_bytes * n);
   void* b = B.get(key_bytes * n);
-  void* tmp = B.get(radix_sort_tmp_bytes(blocks));
+  void* tmp = B.get(radix_sort_tmp_bytes(n));
   unsigned long long* cnt = (unsigned long long*)B.get(8);
   if (!a || !b || !tmp || !cnt) return SVO_E_OOM;
   hipStream_t s;

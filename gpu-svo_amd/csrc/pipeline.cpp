@@ -441,8 +441,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     const uint64_t b = (cap_elems + per_block - 1) / per_block;
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, max_blocks));
   };
-  const uint32_t sort_blocks = grid_for(key_cap, 16384, C.tuning.sort_blocks);
-  SVO_TRY(W.get("sort_tmp", radix_sort_tmp_bytes(C.tuning.sort_blocks), &sort_tmp));
+  SVO_TRY(W.get("sort_tmp", radix_sort_tmp_bytes(key_cap), &sort_tmp));
   SVO_TRY(W.get("lvl_tmp", level_tmp_bytes(C.tuning.scan_blocks), &lvl_tmp));
   // level capacities: cap_1 = box cells (>= hits >= unique voxels);
   // cap_l = min(cap_{l-1}, number of level-l cells over the batch)
@@ -497,7 +496,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
                      d_cnt + 3, n_big, s);
   timer.mark("voxelize");
   K* sorted = nullptr;
-  launch_radix_sort<K>(keys_a, keys_b, d_cnt + 2, key_cap, plan.cb + plan.mb, sort_blocks,
+  launch_radix_sort<K>(keys_a, keys_b, d_cnt + 2, key_cap, plan.cb + plan.mb, C.tuning.sort_blocks,
                        sort_tmp, &sorted, s);
   timer.mark("sort");
   launch_level_first<K>(sorted, d_cnt + 2, kl, lv[1], lv[2], d_counts, d_spans, NL,

@@ -9,8 +9,9 @@
 // workers run concurrently on the device.  Finished trees stay resident until
 // the caller frees them; the budget check sees that through the context's
 // byte counter.  A batch that still runs out of memory goes back to the head
-// of the queue and waits for memory to be released, unless nothing else holds
-// memory, in which case its requests fail with SVO_E_OOM.
+// of the queue and is retried while its oldest request is less than a second
+// old (other batches or freed trees may release memory meanwhile); after that
+// its requests complete with SVO_E_OOM.
 #include <chrono>
 #include <deque>
 #include <thread>

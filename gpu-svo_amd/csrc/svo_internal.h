@@ -13,7 +13,7 @@ namespace svo {
 struct Tuning {
   uint32_t small_max_cells = 512;   // bbox cells handled by one lane
   uint32_t chunk_cells = 4096;      // cells per big-triangle work item
-  uint32_t sort_blocks = 1024;      // fixed grid of the radix sort
+  uint32_t sort_blocks = 1024;      // persistent grid bound of the radix sort passes
   uint32_t scan_blocks = 1024;      // fixed grid of the level / attribute scans
 };
 
@@ -37,9 +37,10 @@ void launch_voxelize(const MeshDev* meshes, const MeshParams* params, const uint
 
 // ------------------------------------------------------------ k_sort.hip
 // LSD radix sort of n (device-resident count) keys on bits [0, bits).
-// Returns in *result which of (a, b) holds the sorted keys.  `tmp` needs
-// radix_sort_tmp_bytes(blocks) bytes.
-size_t radix_sort_tmp_bytes(uint32_t blocks);
+// Returns in *result which of (a, b) holds the sorted keys.  n must not exceed
+// cap; `tmp` needs radix_sort_tmp_bytes(cap) bytes.  `blocks` bounds the
+// persistent grid of the digit passes.
+size_t radix_sort_tmp_bytes(uint64_t cap);
 template <typename K>
 void launch_radix_sort(K* a, K* b, const unsigned long long* n, uint64_t cap, int bits,
                        uint32_t blocks, void* tmp, K** result, hipStream_t s);

@@ -60,6 +60,15 @@ std::string SparseVoxelOctree::writeTree(const std::string& base_filename,
   return path;
 }
 
+std::string SparseVoxelOctree::writeVoxel(const std::string& base_filename,
+                                          const std::string& dir) {
+  if (!tree_) throw SvoError(SVO_E_INVALID, "writeVoxel before create");
+  char path[4096];
+  check(svo_write_voxels_obj(tree_, dir.c_str(), base_filename.c_str(), path, sizeof path),
+        "writing the voxels");
+  return path;
+}
+
 std::vector<svo_node_rec> SparseVoxelOctree::nodes() const {
   std::vector<svo_node_rec> v(info_.n_nodes);
   check(svo_tree_copy_field(tree_, SVO_FIELD_NODES_AOS, v.data(), v.size() * sizeof(svo_node_rec)),

@@ -37,6 +37,8 @@ class SparseVoxelOctree {
   void create();
   // SVO.cu:737-753: writes <base_filename>_<depth>_tree.obj; returns the path.
   std::string writeTree(const std::string& base_filename, const std::string& dir = "");
+  // writeVoxel (SVO.cu:755-777): <base_filename>_1_voxel.obj
+  std::string writeVoxel(const std::string& base_filename, const std::string& dir = "");
 
   int depth() const { return info_.depth; }
   const svo_tree_info& info() const { return info_; }

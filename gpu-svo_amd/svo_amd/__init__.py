@@ -107,6 +107,8 @@ _SIGS = {
     "svo_tree_copy_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]),
     "svo_tree_device_ptr": (C.c_void_p, [C.c_void_p, C.c_int]),
     "svo_write_obj": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]),
+    "svo_write_voxels_obj": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                       C.c_size_t]),
     "svo_query_neighbors": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64,
                                       C.c_void_p]),
     "svo_tree_free": (None, [C.c_void_p]),
@@ -291,6 +293,13 @@ class Tree:
         buf = C.create_string_buffer(4096)
         _check(lib().svo_write_obj(self._h, directory.encode(), stem.encode(), buf, 4096),
                "write obj")
+        return buf.value.decode()
+
+    def write_voxels_obj(self, directory: str, stem: str) -> str:
+        """writeVoxel: <directory>/<stem>_1_voxel.obj, one cube per occupied voxel."""
+        buf = C.create_string_buffer(4096)
+        _check(lib().svo_write_voxels_obj(self._h, directory.encode(), stem.encode(), buf, 4096),
+               "write voxel obj")
         return buf.value.decode()
 
     def query_neighbors(self, nodes, slots) -> np.ndarray:

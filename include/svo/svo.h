@@ -19,6 +19,7 @@
  *   private nodeEdgeArray     SVO.h:51            svo_tree_copy_field(EDGES)
  *   private depthNumNodes     SVO.h:44            svo_tree_info.level_nodes
  *   SparseVoxelOctree::writeTree SVO.cu:737-753   svo_write_obj
+ *   SparseVoxelOctree::writeVoxel SVO.cu:755-777  svo_write_voxels_obj
  *   main()                    main.cpp:7-17       the `svo` CLI (host/svo_main.cpp)
  *
  * Conventions: every function returns an svo_status (0 = success, negative =
@@ -199,6 +200,12 @@ const void* svo_tree_device_ptr(const svo_tree* tree, int field);
  * returned in path_out (may be NULL). */
 int svo_write_obj(const svo_tree* tree, const char* dir, const char* stem, char* path_out,
                   size_t path_cap);
+/* writeVoxel, SVO.cu:755-777 (which create() calls in NDEBUG builds,
+ * SVO.cu:411-419, always with the stem "bunny"): writes
+ * <dir>/<stem>_1_voxel.obj, one cube per occupied voxel in Morton order at
+ * bbox_min + unit[0] * (x, y, z) with edge unit[0], in the writeTree format. */
+int svo_write_voxels_obj(const svo_tree* tree, const char* dir, const char* stem, char* path_out,
+                         size_t path_cap);
 /* Neighbor lookup on a resident tree: out[i] = neighbors[node[i]][slot[i]].
  * node, slot and out are host arrays. */
 int svo_query_neighbors(const svo_tree* tree, const uint32_t* node, const uint8_t* slot,

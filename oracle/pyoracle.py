@@ -66,6 +66,7 @@ def lib():
         L.oracle_tree_field_bytes.restype = C.c_uint64
         L.oracle_tree_get_field.argtypes = [P, C.c_int32, P, C.c_uint64]
         L.oracle_write_obj.argtypes = [P, C.c_char_p]
+        L.oracle_write_voxel_obj.argtypes = [P, C.c_char_p]
         L.oracle_morton_encode_u16.argtypes = [C.c_uint32] * 3
         L.oracle_morton_encode_u16.restype = C.c_uint32
         L.oracle_morton_encode_u64.argtypes = [C.c_uint32] * 3
@@ -76,6 +77,7 @@ def lib():
         L.oracle_voxelize.argtypes = [P, C.c_uint64, P, C.c_uint64, P, C.c_int32, P, C.c_uint64,
                                       C.POINTER(C.c_int32)]
         L.oracle_voxelize.restype = C.c_uint64
+        L.oracle_set_arith_variant.argtypes = [C.c_int32]
         _lib = L
     return _lib
 
@@ -168,7 +170,8 @@ def build(points, tris, res, semantics=SEM_REFERENCE, key_bits=32, threads=1, ke
             L.oracle_tree_free(h)
 
 
-def write_obj(points, tris, res, path, semantics=SEM_REFERENCE, key_bits=32):
+def write_obj(points, tris, res, path, semantics=SEM_REFERENCE, key_bits=32, voxels=False):
+    """writeTree (or writeVoxel with voxels=True) to `path`; returns (status, depth)."""
     L = lib()
     pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
     tri = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
@@ -181,7 +184,8 @@ def write_obj(points, tris, res, path, semantics=SEM_REFERENCE, key_bits=32):
             return st.value, 0
         info = OracleInfo()
         L.oracle_tree_info(h, C.byref(info))
-        return L.oracle_write_obj(h, path.encode()), info.depth
+        w = L.oracle_write_voxel_obj if voxels else L.oracle_write_obj
+        return w(h, path.encode()), info.depth
     finally:
         L.oracle_tree_free(h)
 
@@ -199,6 +203,11 @@ def voxelize(points, tris, res, key_bits=32):
         L.oracle_voxelize(_ptr(pts), pts.shape[0], _ptr(tri), tri.shape[0], _ptr(r), key_bits,
                           _ptr(out), n, C.byref(st))
     return st.value, out
+
+
+def set_arith_variant(variant: int):
+    """0 = pinned restatement; bit 0 left-to-right 3-term dots; bit 1 FMA contraction."""
+    lib().oracle_set_arith_variant(int(variant))
 
 
 def morton_encode_u16(x, y, z):

@@ -184,9 +184,30 @@ struct V3 {
   float x, y, z;
 };
 inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+// Arithmetic variant, for measuring how sensitive the voxel set is to what
+// cannot be pinned here (the default, 0, is the pinned restatement):
+//   bit 0  length-3 sums as (a0 + a1) + a2 instead of Eigen's a0 + (a1 + a2)
+//   bit 1  products feeding an add contracted to fused multiply-adds, as
+//          nvcc does by default (-fmad=true) in dots and the cross product
+int g_arith = 0;
+
 // Eigen 3.4 fixed-size redux splits a length-3 sum as a0 + (a1 + a2).
-inline float dot3(V3 a, V3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
-inline float dot2(float a0, float a1, float b0, float b1) { return a0 * b0 + a1 * b1; }
+inline float dot3(V3 a, V3 b) {
+  if (__builtin_expect(g_arith == 0, 1)) return a.x * b.x + (a.y * b.y + a.z * b.z);
+  if (g_arith & 2) {
+    if (g_arith & 1) return std::fma(a.z, b.z, std::fma(a.y, b.y, a.x * b.x));
+    return std::fma(a.x, b.x, std::fma(a.y, b.y, a.z * b.z));
+  }
+  return (a.x * b.x + a.y * b.y) + a.z * b.z;
+}
+inline float dot2(float a0, float a1, float b0, float b1) {
+  if (__builtin_expect(g_arith & 2, 0)) return std::fma(a0, b0, a1 * b1);
+  return a0 * b0 + a1 * b1;
+}
+inline float cross_c(float a, float b, float c, float d) {
+  if (__builtin_expect(g_arith & 2, 0)) return std::fma(a, b, -(c * d));
+  return a * b - c * d;
+}
 
 struct Grid {
   float origin[3];  // modelBBox.min
@@ -246,8 +267,9 @@ inline void voxelize_triangle(const Grid& g, V3 t0, V3 t1, V3 t2, Emit&& emit) {
   const V3 v0 = sub(t0, origin), v1 = sub(t1, origin), v2 = sub(t2, origin);  // 58-63
   const V3 e0 = sub(v1, v0), e1 = sub(v2, v1), e2 = sub(v0, v2);              // 65-67
   // e0.cross(e1).normalized(), 69
-  V3 n = {e0.y * e1.z - e0.z * e1.y, e0.z * e1.x - e0.x * e1.z, e0.x * e1.y - e0.y * e1.x};
-  const float zz = n.x * n.x + (n.y * n.y + n.z * n.z);
+  V3 n = {cross_c(e0.y, e1.z, e0.z, e1.y), cross_c(e0.z, e1.x, e0.x, e1.z),
+          cross_c(e0.x, e1.y, e0.y, e1.x)};
+  const float zz = dot3(n, n);
   if (zz > 0.0f) {
     const float s = std::sqrt(zz);
     n = {n.x / s, n.y / s, n.z / s};
@@ -700,16 +722,16 @@ int build_tree(const float* P, uint64_t np, const int32_t* Ti, uint64_t nt, cons
 // write_vertex / write_line / write_cube, IO.h:8-90, and writeTree,
 // SVO.cu:737-753: ostream output of float is printf "%g" of the value
 // promoted to double; every line ends with std::endl.
-int write_tree(const Tree& T, const char* path) {
+// write_cube, IO.h:24-93, for each of N cubes in order.
+int write_cubes(const float* origin, const float* width, uint64_t N, const char* path) {
   FILE* f = std::fopen(path, "wb");
   if (!f) return ORC_E_IO;
   std::vector<char> buf(1 << 20);
   std::setvbuf(f, buf.data(), _IOFBF, buf.size());
   uint64_t base = 0;
-  const uint64_t N = T.code.size();
   for (uint64_t t = 0; t < N; ++t) {
-    const float* o = &T.origin[3 * t];
-    const float w = T.width[t];
+    const float* o = &origin[3 * t];
+    const float w = width[t];
     // v1..v8 of IO.h:133-140: origin + (0|w per axis)
     static const int C[8][3] = {{0, 1, 1}, {0, 0, 1}, {0, 1, 0}, {0, 0, 0},
                                 {1, 0, 0}, {1, 1, 0}, {1, 0, 1}, {1, 1, 1}};
@@ -726,6 +748,25 @@ int write_tree(const Tree& T, const char* path) {
     base += 8;
   }
   return std::fclose(f) == 0 ? ORC_OK : ORC_E_IO;
+}
+
+// writeTree, SVO.cu:737-753: every node in array order.
+int write_tree(const Tree& T, const char* path) {
+  return write_cubes(T.origin.data(), T.width.data(), T.code.size(), path);
+}
+
+// writeVoxel, SVO.cu:755-777: one cube per voxel of S1 in sorted order at
+// modelBBox.min + unitNodeWidth * (x, y, z), edge unitNodeWidth (= unit.x).
+int write_voxels(const Tree& T, const char* path) {
+  const uint64_t U = T.voxels.size();
+  const float w = T.info.unit[0];
+  std::vector<float> org(3 * U), wid(U, w);
+  for (uint64_t i = 0; i < U; ++i) {
+    uint32_t c[3];
+    decode(T.voxels[i], c[0], c[1], c[2]);
+    for (int a = 0; a < 3; ++a) org[3 * i + a] = T.info.bbox_min[a] + w * (float)c[a];
+  }
+  return write_cubes(org.data(), wid.data(), U, path);
 }
 
 // 168-byte SVONode record, SVO.h:20-35.
@@ -886,6 +927,9 @@ int oracle_tree_get_field(const void* tree, int32_t field, void* dst, uint64_t d
 }
 
 int oracle_write_obj(const void* tree, const char* path) { return write_tree(*(const Tree*)tree, path); }
+int oracle_write_voxel_obj(const void* tree, const char* path) {
+  return write_voxels(*(const Tree*)tree, path);
+}
 
 uint32_t oracle_morton_encode_u16(uint32_t x, uint32_t y, uint32_t z) {
   return encode_u16((uint16_t)x, (uint16_t)y, (uint16_t)z);
@@ -908,6 +952,8 @@ void oracle_luts(uint32_t* enc_x256, uint32_t* enc_y256, uint32_t* enc_z256, uin
   std::memcpy(vert_shared_8x8, L.vert_shared, sizeof(L.vert_shared));
   std::memcpy(edge_shared_12x4, L.edge_shared, sizeof(L.edge_shared));
 }
+
+void oracle_set_arith_variant(int32_t variant) { g_arith = variant & 3; }
 
 uint64_t oracle_voxelize(const float* points, uint64_t n_points, const int32_t* tris,
                          uint64_t n_tris, const int32_t* res, int32_t key_bits, uint64_t* dst,

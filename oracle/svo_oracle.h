@@ -14,6 +14,7 @@
  *   SVO.cu:522-565          neighbors                     -> oracle_build
  *   SVO.cu:567-728          vertices / edges + unique     -> oracle_build
  *   SVO.cu:737-753, IO.h    <stem>_<depth>_tree.obj       -> oracle_write_obj
+ *   SVO.cu:755-777, IO.h    <stem>_1_voxel.obj            -> oracle_write_voxel_obj
  *   MortonLUT.h             encode / decode / neighbor LUTs
  * Floating point follows the reference's operation order with no contraction
  * (built with -ffp-contract=off).
@@ -90,6 +91,9 @@ int oracle_tree_info(const void* tree, oracle_info* out);
 uint64_t oracle_tree_field_bytes(const void* tree, int32_t field);
 int oracle_tree_get_field(const void* tree, int32_t field, void* dst, uint64_t dst_bytes);
 int oracle_write_obj(const void* tree, const char* path);
+/* writeVoxel, SVO.cu:755-777: cubes of the occupied voxels (the file the
+ * reference writes as bunny_1_voxel.obj in NDEBUG builds). */
+int oracle_write_voxel_obj(const void* tree, const char* path);
 
 /* Building blocks exposed for closed-form tests. */
 uint32_t oracle_morton_encode_u16(uint32_t x, uint32_t y, uint32_t z); /* MortonLUT.h:482-501 */
@@ -99,6 +103,11 @@ void oracle_luts(uint32_t* enc_x256, uint32_t* enc_y256, uint32_t* enc_z256, uin
                  uint8_t* dec_y512, uint8_t* dec_z512, int16_t* nb_parent_8x27,
                  int16_t* nb_child_8x27, int16_t* vert_shared_8x8, int16_t* edge_shared_12x4);
 /* Voxelize only (sorted unique codes); returns count, writes up to cap codes. */
+// Selects an arithmetic variant of the voxel tests for sensitivity studies:
+// bit 0 sums 3-term dots left to right, bit 1 contracts products into fused
+// multiply-adds.  0 (the default) is the restatement every test pins.
+void oracle_set_arith_variant(int32_t variant);
+
 uint64_t oracle_voxelize(const float* points, uint64_t n_points, const int32_t* tris,
                          uint64_t n_tris, const int32_t* res, int32_t key_bits, uint64_t* dst,
                          uint64_t cap, int32_t* status);

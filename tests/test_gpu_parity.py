@@ -116,3 +116,26 @@ def test_statuses(gpu_ctx):
     with pytest.raises(svo_amd.SvoError) as e:
         gpu_ctx.build(P2, T, 16)
     assert e.value.status == svo_amd.E_INVALID
+
+
+@pytest.mark.parametrize("res,kb", [(64, 32), ((40, 24, 33), 32), (128, 64)])
+def test_obj_files_byte_exact(gpu_ctx, bunny, tmp_path, res, kb):
+    """writeTree and writeVoxel files from the GPU tree equal the oracle's bytes."""
+    P, T = bunny
+    print(f"obj file case: bunny res {res} key_bits {kb}", flush=True)
+    tree = gpu_ctx.build(P, T, res, key_bits=kb)
+    d = tmp_path / "gpu"
+    d.mkdir()
+    path = tree.write_obj(str(d), "bunny")
+    vpath = tree.write_voxels_obj(str(d), "bunny")
+    depth = tree.info.depth
+    assert path.endswith(f"bunny_{depth}_tree.obj") and vpath.endswith("bunny_1_voxel.obj")
+    ref_t = str(tmp_path / "ref_tree.obj")
+    ref_v = str(tmp_path / "ref_voxel.obj")
+    st, rdepth = po.write_obj(P, T, res, ref_t, key_bits=kb)
+    assert st == 0 and rdepth == depth
+    st, _ = po.write_obj(P, T, res, ref_v, key_bits=kb, voxels=True)
+    assert st == 0
+    assert open(path, "rb").read() == open(ref_t, "rb").read()
+    assert open(vpath, "rb").read() == open(ref_v, "rb").read()
+    tree.free()

@@ -131,3 +131,23 @@ def test_oracle_writer_format(tmp_path):
     assert st == 0
     r = po.build(P, T, 16)
     assert open(path).read() == expected_tree_text(r["origin"], r["width"])
+
+
+def test_oracle_voxel_writer_is_level1_cubes(tmp_path):
+    # writeVoxel's cube for voxel c is bboxMin + unit.x * decode(c) with edge
+    # unit.x, which is the level-1 node of code c: the file must equal the
+    # tree writer's text for the level-1 nodes that are occupied voxels.
+    from svo_amd import meshgen
+    P, T = meshgen.torus(0.4, 0.12, 24, 12)
+    for res in (16, (20, 12, 9)):
+        path = str(tmp_path / "t_1_voxel.obj")
+        st, _ = po.write_obj(P, T, res, path, voxels=True)
+        assert st == 0
+        r = po.build(P, T, res)
+        n1 = int(r.info.level_nodes[0])
+        code1 = r["code"][:n1]
+        occ = np.isin(code1, r["voxels"])
+        assert occ.sum() == r["voxels"].size
+        assert np.array_equal(code1[occ], r["voxels"])
+        exp = expected_tree_text(r["origin"][:n1][occ], r["width"][:n1][occ])
+        assert open(path).read() == exp
