@@ -99,16 +99,20 @@ def cpu_baseline(meshes, budget_s=12.0):
         done += 1
         if t1 > budget_s:
             break
+    # all host threads: meshes are independent, so each thread builds whole
+    # meshes (the oracle releases the interpreter lock while it runs)
+    from concurrent.futures import ThreadPoolExecutor
     nt = cpu_threads()
-    tn = 0.0
-    for P, T, res in meshes[:done]:
-        r = po.build(P, T, res, threads=nt, fields=["voxels"])
-        tn += r.info.build_seconds
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(nt) as ex:
+        list(ex.map(lambda m: po.build(m[0], m[1], m[2], threads=1, fields=["voxels"]),
+                    meshes[:done]))
+    tn = time.perf_counter() - t0
     return {"value": round(done / tn, 3), "unit": "meshes/s", "cores": nt, "kind": "port",
             "single_thread_value": round(done / t1, 3),
             "sample": f"first {done} meshes of the same 256-mesh set, build only "
-                      f"(voxelize -> vertex/edge): {t1:.1f} s on 1 thread, {tn:.1f} s on "
-                      f"{nt} threads"}
+                      f"(voxelize -> vertex/edge): {t1:.1f} s on 1 thread, {tn:.1f} s wall "
+                      f"on {nt} threads (one mesh per thread)"}
 
 
 def run_batch_workload(args):
@@ -370,6 +374,22 @@ def run_served_workload(args):
     }
 
 
+def copy_probe(nbytes):
+    """One device-to-device copy of nbytes (reads nbytes, writes nbytes)."""
+    import ctypes as C
+    import svo_amd
+    L = svo_amd.lib()
+    ctx = svo_amd.Context(0)
+    a, b = C.c_void_p(), C.c_void_p()
+    svo_amd._check(L.svo_device_alloc(ctx.handle, nbytes, C.byref(a)), "probe alloc")
+    svo_amd._check(L.svo_device_alloc(ctx.handle, nbytes, C.byref(b)), "probe alloc")
+    svo_amd._check(L.svo_device_copy(ctx.handle, b, a, nbytes, 2), "probe copy")
+    L.svo_device_free(ctx.handle, a, nbytes)
+    L.svo_device_free(ctx.handle, b, nbytes)
+    ctx.close()
+    log(f"bench: copy probe of {nbytes} bytes done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,6 +402,9 @@ def main():
     ap.add_argument("--max-batch", type=int, default=32)
     ap.add_argument("--max-wait-us", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first copy this many MiB device-to-device once: a kernel of known "
+                         "traffic for scaling HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -389,6 +412,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     from svo_amd import meshgen
+    if args.copy_probe_mb > 0:
+        copy_probe(args.copy_probe_mb << 20)
     if args.workload == "batch":
         out = run_batch_workload(args)
     elif args.workload == "single":

@@ -105,6 +105,10 @@ __device__ inline GroupNode<K> group_node(int level, KeyLayout<K> kl, const K* e
   return r;
 }
 
+// One thread per node, eight per parent group.  The loads are issued in two
+// rounds that do not depend on each other within a round (group arrays, then
+// everything keyed by the mesh and the first child), so a wave waits for two
+// memory latencies before it stores.
 template <typename K>
 __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
     int level, KeyLayout<K> kl, const K* elem_l, uint64_t n_elem_l, const K* elem_next,
@@ -112,46 +116,59 @@ __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
     const LevelSpan* spans, uint32_t n_levels, const MeshTreeState* state,
     const MeshNodeLayout* layout, const MeshParams* params, NodeArrays out) {
   const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-  const uint64_t g = gid >> 3;
+  const uint64_t g0 = gid >> 3;
   const uint32_t k = (uint32_t)(gid & 7);
-  const GroupNode<K> gn = group_node(level, kl, elem_next, group_next, n_groups, spans, n_levels,
-                                     state, layout, g, k);
-  // occupancy mask of the group: lane k of the group looks at member k
-  uint32_t fc = 0, bit = 0;
-  if (gn.live) {
-    fc = first_child_next[g];
-    const uint64_t j = (uint64_t)fc + k;
-    if (j < n_elem_l) {
-      const K e = elem_l[j];
-      if (kl.parent(e) == gn.P) bit = 1u << (uint32_t)(e & 7);
-    }
-  }
+  const bool in = g0 < n_groups;
+  const uint64_t g = in ? g0 : 0;
+  // round 1: the group's parent key, first child and parent group
+  const K P = elem_next[g];
+  const uint32_t fc = first_child_next[g];
+  const uint32_t gnext = group_next[g];
+  // round 2: mesh state, layout and spans; member k of the group
+  const uint32_t m = kl.mesh(P);
+  const uint32_t depth = state[m].depth;
+  const MeshNodeLayout* L = layout + m;
+  const uint64_t nn = L->n_nodes, nbase = L->node_base;
+  const uint64_t off_l = L->level_off[level - 1], off_p = L->level_off[level];
+  const uint64_t off_c = level >= 2 ? L->level_off[level - 2] : 0;
+  const float w = L->width[level - 1];
+  const LevelSpan sp = spans[(uint64_t)m * n_levels + (level - 1)];
+  const uint32_t spn_first = spans[(uint64_t)m * n_levels + level].group_first;
+  const float ox = params[m].origin[0], oy = params[m].origin[1], oz = params[m].origin[2];
+  const uint64_t j = (uint64_t)fc + k;
+  const K e = j < n_elem_l ? elem_l[j] : (K)0;
+  // a depth-1 tree's level-1 group is no node group; a mesh rejected after the
+  // level phase has no nodes
+  const bool live = in && depth > (uint32_t)level && nn != 0;
+  uint32_t bit = (live && j < n_elem_l && kl.parent(e) == P) ? 1u << (uint32_t)(e & 7) : 0u;
   bit |= __shfl_xor(bit, 1);
   bit |= __shfl_xor(bit, 2);
   bit |= __shfl_xor(bit, 4);
-  if (!gn.live) return;
+  if (!live) return;
   const uint32_t mask = bit;
   const bool real = (mask >> k) & 1u;
-  const uint32_t m = gn.m;
-  const MeshNodeLayout& L = layout[m];
-  const uint64_t t = L.node_base + gn.local;
-  const K code = (kl.code(gn.P) << 3) | (K)k;
-  const float w = L.width[level - 1];
-  float o[3];
-  decode_origin<K>(params[m], w, code, o);
+  const uint64_t local = off_l + 8ull * (g - sp.group_first) + k;
+  const uint64_t t = nbase + local;
+  const uint32_t parent = depth == (uint32_t)level + 1
+                              ? (uint32_t)off_p  // the root
+                              : (uint32_t)(off_p + 8ull * (gnext - spn_first) +
+                                           (uint32_t)(P & 7));
+  const K code = (kl.code(P) << 3) | (K)k;
+  uint32_t x, y, z;
+  demorton3<K>(code, x, y, z);
   store_code(out, t, (uint64_t)code);
   out.is_leaf[t] = (level == 1 || !real) ? 1 : 0;
-  out.origin[3 * t + 0] = o[0];
-  out.origin[3 * t + 1] = o[1];
-  out.origin[3 * t + 2] = o[2];
+  // gridOrigin + width * Vector3f(x, y, z), per axis (SVO.cu:262)
+  out.origin[3 * t + 0] = ox + w * (float)x;
+  out.origin[3 * t + 1] = oy + w * (float)y;
+  out.origin[3 * t + 2] = oz + w * (float)z;
   out.width[t] = w;
-  out.parent[t] = gn.parent;
+  out.parent[t] = parent;
   uint32_t cbase = 0;
   const bool has_childs = real && level >= 2;
   if (has_childs) {
-    const LevelSpan sp = spans[(uint64_t)m * n_levels + (level - 1)];
     const uint64_t i = (uint64_t)fc + __popc(mask & ((1u << k) - 1u));
-    cbase = (uint32_t)(L.level_off[level - 2] + 8ull * (i - sp.elem_first));
+    cbase = (uint32_t)(off_c + 8ull * (i - sp.elem_first));
   }
   store_childs(out.childs + 8 * t, cbase, has_childs);
 }

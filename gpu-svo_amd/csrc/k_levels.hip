@@ -6,7 +6,7 @@
 // SVO.cu:237-309), fills the empty siblings (createRemainNode, 311-339) and
 // stops at the first level holding a single code (487-490).
 //
-// Here every level is one scan over the level's sorted unique keys:
+// Here every level is one pass over the level's sorted unique keys:
 //   group head  = first key of its parent group (parent differs from the
 //                 previous key's parent; composite keys keep meshes apart),
 //   root        = the only key of its mesh at this level (the reference's
@@ -15,8 +15,16 @@
 //                 unique) and, per group, the index of its first child.
 // Level 1 is fused with the de-duplication of the sorted hit keys.  Node
 // records are written later from these arrays (k_attrs.hip), once the node
-// counts are known.  Each scan is count -> block-offset scan -> write over a
-// fixed grid, with the element count read from device memory.
+// counts are known.
+//
+// Each pass is a single-pass scan: persistent workgroups claim 4096-element
+// tiles by ticket, each thread loads 16 consecutive keys with vector loads,
+// the tile's counts are scanned in the workgroup and its offset found by
+// decoupled look-back (svo_scan.h); compacted outputs are staged in LDS and
+// written contiguously.  The element count is read from device memory, so no
+// launch shape depends on it.
+#include <algorithm>
+
 #include "svo_internal.h"
 #include "svo_scan.h"
 
@@ -24,30 +32,12 @@ namespace svo {
 namespace {
 
 constexpr int BLOCK = 256;
-constexpr int EPT = 8;  // elements per thread per tile
-constexpr int TILE = BLOCK * EPT;
+constexpr int EPT = 16;             // consecutive elements per thread
+constexpr int TILE = BLOCK * EPT;   // 4096
+constexpr int MAX_TICKETS = 64;     // one per pass: level 1 .. level 63
 
-__device__ inline void block_range(uint64_t n, uint32_t G, uint32_t b, uint64_t& beg,
-                                   uint64_t& end) {
-  const uint64_t chunk = (n + G - 1) / G;
-  beg = (uint64_t)b * chunk;
-  if (beg > n) beg = n;
-  end = beg + chunk;
-  if (end > n) end = n;
-}
-
-// Loads keys [tb - 1, tb + tn] into s[0 .. tn + 1] with coalesced reads, so
-// that every element of the tile sees its neighbours in LDS.
-template <typename K>
-__device__ inline void load_tile(const K* src, uint64_t n, uint64_t tb, uint32_t tn, K* s) {
-  __syncthreads();  // the previous tile is no longer read
-  for (uint32_t j = threadIdx.x; j < tn + 2; j += BLOCK) {
-    const uint64_t gi = tb + j;  // element gi - 1
-    s[j] = (gi >= 1 && gi - 1 < n) ? src[gi - 1] : (K)0;
-  }
-  __syncthreads();
-}
-
+// flags of sorted hit key `cur`: bit0 = first copy of its key, bit1 = head of
+// its parent group
 template <typename K>
 __device__ inline uint32_t first_flags_v(K prev, K cur, bool has_prev, KeyLayout<K> kl) {
   if (!has_prev) return 3u;
@@ -55,11 +45,12 @@ __device__ inline uint32_t first_flags_v(K prev, K cur, bool has_prev, KeyLayout
   return 1u | (kl.parent(cur) != kl.parent(prev) ? 2u : 0u);
 }
 
+// flags of level element `cur` of a mesh still being built: bit1 = group
+// head, bit2 = the only element of its mesh (the root)
 template <typename K>
 __device__ inline uint32_t level_flags_v(K prev, K cur, K next, bool has_prev, bool has_next,
-                                         KeyLayout<K> kl, const MeshTreeState* state) {
+                                         KeyLayout<K> kl) {
   const uint32_t m = kl.mesh(cur);
-  if (state[m].depth != 0) return 0u;
   const bool first = !has_prev || kl.mesh(prev) != m;
   const bool last = !has_next || kl.mesh(next) != m;
   if (first && last) return 4u;
@@ -67,131 +58,162 @@ __device__ inline uint32_t level_flags_v(K prev, K cur, K next, bool has_prev, b
   return kl.parent(cur) != kl.parent(prev) ? 2u : 0u;
 }
 
+__global__ __launch_bounds__(BLOCK) void k_level_clear(uint64_t* status, uint32_t* tickets,
+                                                       const unsigned long long* n_ptr) {
+  const uint64_t words = 2 * ((*n_ptr + TILE - 1) / TILE);
+  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < words; i += stride)
+    status[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < MAX_TICKETS) tickets[threadIdx.x] = 0;
+}
+
+// Per-tile frame shared by both passes: claims a tile, loads the thread's run
+// and the keys just before and after it.
+template <typename K>
+struct TileRun {
+  uint64_t tile, tb, i0;
+  uint32_t tn, cnt;
+  K kk[EPT];
+  K prev, next;
+  bool has_prev, has_next;
+};
+
+template <typename K>
+__device__ inline bool claim_tile(const K* src, uint64_t n, uint32_t* ticket, uint32_t* s_tile,
+                                  K (*s_edge)[BLOCK], TileRun<K>& r) {
+  if (threadIdx.x == 0) *s_tile = atomicAdd(ticket, 1u);
+  __syncthreads();
+  r.tile = *s_tile;
+  r.tb = r.tile * TILE;
+  if (r.tb >= n) return false;  // uniform over the workgroup
+  r.tn = (uint32_t)((n - r.tb) < TILE ? (n - r.tb) : TILE);
+  const uint32_t l0 = threadIdx.x * EPT;
+  r.cnt = l0 < r.tn ? ((r.tn - l0) < EPT ? (r.tn - l0) : EPT) : 0u;
+  r.i0 = r.tb + l0;
+  load_consecutive(src + r.i0, ((uintptr_t)src & 15) == 0, r.cnt, r.kk);
+  K last = r.kk[0];
+#pragma unroll
+  for (int e = 1; e < EPT; ++e)
+    if ((uint32_t)e < r.cnt) last = r.kk[e];
+  s_edge[0][threadIdx.x] = r.kk[0];
+  s_edge[1][threadIdx.x] = last;
+  __syncthreads();
+  r.has_prev = r.cnt && r.i0 > 0;
+  r.prev = !r.has_prev ? (K)0 : threadIdx.x > 0 ? s_edge[1][threadIdx.x - 1] : src[r.i0 - 1];
+  r.has_next = r.cnt && r.i0 + r.cnt < n;
+  r.next = !r.has_next ? (K)0
+           : (threadIdx.x + 1 < BLOCK && l0 + EPT < r.tn) ? s_edge[0][threadIdx.x + 1]
+                                                           : src[r.i0 + r.cnt];
+  return true;
+}
+
 // ---------------------------------------------------------------- level 1
-// flags of sorted hit key i: bit0 = first copy of its key, bit1 = parent-group head
+// Sorted hit keys -> unique keys (elem1) with their group (group1), the level-2
+// keys (elem2 = group parents) with their first child (first_child2), the
+// per-mesh spans of level 1, and counts[0] / counts[1].
 template <typename K>
-__device__ inline uint32_t first_flags(const K* k, uint64_t i, KeyLayout<K> kl) {
-  const K cur = k[i];
-  if (i == 0) return 3u;
-  const K prev = k[i - 1];
-  if (cur == prev) return 0u;
-  return 1u | (kl.parent(cur) != kl.parent(prev) ? 2u : 0u);
-}
-
-template <typename K>
-__global__ __launch_bounds__(BLOCK) void k_first_count(const K* sorted,
-                                                       const unsigned long long* n_ptr,
-                                                       KeyLayout<K> kl, uint32_t G,
-                                                       uint64_t* partial) {
+__global__ __launch_bounds__(BLOCK) void k_first_pass(
+    const K* sorted, const unsigned long long* n_ptr, KeyLayout<K> kl, K* elem1,
+    uint32_t* group1, K* elem2, uint32_t* first_child2, LevelSpan* spans, uint32_t n_levels,
+    uint64_t* status, uint32_t* ticket, uint32_t epoch, unsigned long long* counts) {
+  __shared__ K s_edge[2][BLOCK];
+  __shared__ K s_key[TILE];
+  __shared__ uint32_t s_val[TILE];
   __shared__ uint64_t tmp[4];
-  uint64_t beg, end;
-  block_range(*n_ptr, G, blockIdx.x, beg, end);
-  uint64_t nu = 0, ng = 0;
-  for (uint64_t i = beg + threadIdx.x; i < end; i += BLOCK) {
-    const uint32_t f = first_flags(sorted, i, kl);
-    nu += f & 1u;
-    ng += f >> 1;
-  }
-  uint64_t tu, tg;
-  block_excl_scan256(nu, tmp, tu);
-  block_excl_scan256(ng, tmp, tg);
-  if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = tu;
-    partial[2 * blockIdx.x + 1] = tg;
-  }
-}
-
-// Exclusive scan of G pairs (single block); totals to *out0 / *out1 when set.
-__global__ __launch_bounds__(BLOCK) void k_pair_scan(uint64_t* partial, uint32_t G,
-                                                     unsigned long long* out0,
-                                                     unsigned long long* out1) {
-  __shared__ uint64_t tmp[4];
-  const uint32_t per = (G + BLOCK - 1) / BLOCK;
-  const uint32_t b0 = threadIdx.x * per;
-  uint64_t s0 = 0, s1 = 0;
-  for (uint32_t k = 0; k < per; ++k)
-    if (b0 + k < G) {
-      s0 += partial[2 * (b0 + k)];
-      s1 += partial[2 * (b0 + k) + 1];
-    }
-  uint64_t t0, t1;
-  uint64_t r0 = block_excl_scan256(s0, tmp, t0);
-  uint64_t r1 = block_excl_scan256(s1, tmp, t1);
-  for (uint32_t k = 0; k < per; ++k)
-    if (b0 + k < G) {
-      const uint64_t c0 = partial[2 * (b0 + k)], c1 = partial[2 * (b0 + k) + 1];
-      partial[2 * (b0 + k)] = r0;
-      partial[2 * (b0 + k) + 1] = r1;
-      r0 += c0;
-      r1 += c1;
-    }
-  if (threadIdx.x == 0) {
-    if (out0) *out0 = t0;
-    if (out1) *out1 = t1;
-  }
-}
-
-template <typename K>
-__global__ __launch_bounds__(BLOCK) void k_first_write(const K* sorted,
-                                                       const unsigned long long* n_ptr,
-                                                       KeyLayout<K> kl, uint32_t G,
-                                                       const uint64_t* partial, K* elem1,
-                                                       uint32_t* group1, K* elem2,
-                                                       uint32_t* first_child2, LevelSpan* spans,
-                                                       uint32_t n_levels) {
-  __shared__ uint64_t tmp[4];
-  __shared__ K sk[TILE + 2];
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_base[2];
   const uint64_t n = *n_ptr;
-  uint64_t beg, end;
-  block_range(n, G, blockIdx.x, beg, end);
-  uint64_t ubase = partial[2 * blockIdx.x], gbase = partial[2 * blockIdx.x + 1];
-  for (uint64_t tb = beg; tb < end; tb += TILE) {
-    const uint32_t tn = (uint32_t)((end - tb) < TILE ? (end - tb) : TILE);
-    load_tile(sorted, n, tb, tn, sk);
-    // blocked: thread t owns elements tb + t*EPT .. +EPT
-    const uint32_t l0 = threadIdx.x * EPT;
-    const uint64_t i0 = tb + l0;
-    uint32_t f[EPT];
-    uint64_t nu = 0, ng = 0;
+  TileRun<K> r;
+  while (claim_tile(sorted, n, ticket, &s_tile, s_edge, r)) {
+    uint32_t f = 0, nu = 0, ng = 0;
+    {
+      K pk = r.prev;
+      bool hp = r.has_prev;
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      const uint32_t l = l0 + e;
-      f[e] = (l < tn) ? first_flags_v(sk[l], sk[l + 1], tb + l > 0, kl) : 0u;
-      nu += f[e] & 1u;
-      ng += f[e] >> 1;
-    }
-    uint64_t tu, tg;
-    uint64_t ru = block_excl_scan256(nu, tmp, tu) + ubase;  // uniques before i0
-    uint64_t rg = block_excl_scan256(ng, tmp, tg) + gbase;  // groups before i0
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      const uint64_t i = i0 + e;
-      if (i >= end) break;
-      const K key = sk[l0 + e + 1];
-      const uint32_t m = kl.mesh(key);
-      if (f[e] & 1u) {
-        if (f[e] & 2u) {
-          elem2[rg] = kl.parent(key);
-          first_child2[rg] = (uint32_t)ru;
-          ++rg;
+      for (int e = 0; e < EPT; ++e) {
+        if ((uint32_t)e < r.cnt) {
+          const uint32_t fl = first_flags_v(pk, r.kk[e], hp, kl);
+          f |= fl << (2 * e);
+          nu += fl & 1u;
+          ng += fl >> 1;
+          pk = r.kk[e];
+          hp = true;
         }
-        elem1[ru] = key;
-        group1[ru] = (uint32_t)(rg - 1);
-        // first element of its mesh
-        if (i == 0 || kl.mesh(sk[l0 + e]) != m) {
-          spans[(uint64_t)m * n_levels].elem_first = (uint32_t)ru;
-          spans[(uint64_t)m * n_levels].group_first = (uint32_t)(rg - 1);
-        }
-        ++ru;
-      }
-      // last element (copy) of its mesh
-      if (i + 1 == n || kl.mesh(sk[l0 + e + 2]) != m) {
-        spans[(uint64_t)m * n_levels].elem_end = (uint32_t)ru;
-        spans[(uint64_t)m * n_levels].group_end = (uint32_t)rg;
       }
     }
-    ubase += tu;
-    gbase += tg;
+    uint64_t tot;
+    const uint64_t ex = block_excl_scan256(((uint64_t)ng << 32) | nu, tmp, tot);
+    const uint32_t tu = (uint32_t)tot, tg = (uint32_t)(tot >> 32);
+    if (threadIdx.x < 2) {
+      const uint32_t c = threadIdx.x == 0 ? tu : tg;
+      const uint32_t pre = lookback_exclusive(status + threadIdx.x, 2, r.tile, c, epoch);
+      s_base[threadIdx.x] = pre;
+      if (r.tb + r.tn == n) counts[threadIdx.x] = (unsigned long long)pre + c;
+    }
+    __syncthreads();
+    const uint32_t bu = s_base[0], bg = s_base[1];
+    // elem1 / group1 staged by tile-local unique index; spans on the way
+    {
+      uint32_t ru = (uint32_t)ex, rg = (uint32_t)(ex >> 32);
+      K pk = r.prev;
+      bool hp = r.has_prev;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        if ((uint32_t)e < r.cnt) {
+          const K key = r.kk[e];
+          const uint32_t fl = (f >> (2 * e)) & 3u;
+          const uint32_t m = kl.mesh(key);
+          if (fl & 1u) {
+            if (fl & 2u) ++rg;
+            s_key[ru] = key;
+            s_val[ru] = bg + rg - 1;
+            if (!hp || kl.mesh(pk) != m) {  // first element of its mesh
+              spans[(uint64_t)m * n_levels].elem_first = bu + ru;
+              spans[(uint64_t)m * n_levels].group_first = bg + rg - 1;
+            }
+            ++ru;
+          }
+          const bool hn = (uint32_t)e + 1 < r.cnt || r.has_next;
+          const K nk = (uint32_t)e + 1 < r.cnt ? r.kk[e + 1 < EPT ? e + 1 : e] : r.next;
+          if (!hn || kl.mesh(nk) != m) {  // last element (copy) of its mesh
+            spans[(uint64_t)m * n_levels].elem_end = bu + ru;
+            spans[(uint64_t)m * n_levels].group_end = bg + rg;
+          }
+          pk = key;
+          hp = true;
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tu; i += BLOCK) {
+      elem1[(uint64_t)bu + i] = s_key[i];
+      group1[(uint64_t)bu + i] = s_val[i];
+    }
+    __syncthreads();
+    // elem2 / first_child2 staged by tile-local group index
+    {
+      uint32_t ru = (uint32_t)ex, rg = (uint32_t)(ex >> 32);
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        if ((uint32_t)e < r.cnt) {
+          const uint32_t fl = (f >> (2 * e)) & 3u;
+          if (fl & 1u) {
+            if (fl & 2u) {
+              s_key[rg] = kl.parent(r.kk[e]);
+              s_val[rg] = bu + ru;
+              ++rg;
+            }
+            ++ru;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tg; i += BLOCK) {
+      elem2[(uint64_t)bg + i] = s_key[i];
+      first_child2[(uint64_t)bg + i] = s_val[i];
+    }
+    __syncthreads();
   }
 }
 
@@ -207,128 +229,155 @@ __global__ void k_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint
 }
 
 // ---------------------------------------------------------------- level l >= 2
-// flags of element i: bit1 = group head, bit2 = root of its mesh, 0 = skipped
+// Level elements -> their group (group_l, NONE for roots and finished
+// meshes), the next level's keys with their first child, the spans of this
+// level, roots into state, and counts[level] (elements of the next level).
 template <typename K>
-__device__ inline uint32_t level_flags(const K* e, uint64_t i, uint64_t n, KeyLayout<K> kl,
-                                       const MeshTreeState* state) {
-  const K cur = e[i];
-  const uint32_t m = kl.mesh(cur);
-  if (state[m].depth != 0) return 0u;  // finished at a lower level (single voxel)
-  const bool first = (i == 0) || kl.mesh(e[i - 1]) != m;
-  const bool last = (i + 1 == n) || kl.mesh(e[i + 1]) != m;
-  if (first && last) return 4u;
-  if (first) return 2u;
-  return kl.parent(cur) != kl.parent(e[i - 1]) ? 2u : 0u;
-}
-
-template <typename K>
-__global__ __launch_bounds__(BLOCK) void k_level_count(const K* elem,
-                                                       const unsigned long long* n_ptr,
-                                                       KeyLayout<K> kl, const MeshTreeState* state,
-                                                       uint32_t G, uint64_t* partial) {
-  __shared__ uint64_t tmp[4];
-  const uint64_t n = *n_ptr;
-  uint64_t beg, end;
-  block_range(n, G, blockIdx.x, beg, end);
-  uint64_t ng = 0;
-  for (uint64_t i = beg + threadIdx.x; i < end; i += BLOCK)
-    ng += (level_flags(elem, i, n, kl, state) >> 1) & 1u;
-  uint64_t tg;
-  block_excl_scan256(ng, tmp, tg);
-  if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = 0;
-    partial[2 * blockIdx.x + 1] = tg;
-  }
-}
-
-template <typename K>
-__global__ __launch_bounds__(BLOCK) void k_level_write(
+__global__ __launch_bounds__(BLOCK) void k_level_pass(
     int level, const K* elem, const unsigned long long* n_ptr, KeyLayout<K> kl,
-    MeshTreeState* state, uint32_t G, const uint64_t* partial, uint32_t* group_l, K* elem_next,
-    uint32_t* first_child_next, LevelSpan* spans, uint32_t n_levels) {
-  __shared__ uint64_t tmp[4];
-  __shared__ K sk[TILE + 2];
+    MeshTreeState* state, uint32_t* group_l, K* elem_next, uint32_t* first_child_next,
+    LevelSpan* spans, uint32_t n_levels, uint64_t* status, uint32_t* ticket, uint32_t epoch,
+    unsigned long long* count_next) {
+  __shared__ K s_edge[2][BLOCK];
+  __shared__ K s_key[TILE];
+  __shared__ uint32_t s_val[TILE];
+  __shared__ uint32_t tmp[4];
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_base;
   const uint64_t n = *n_ptr;
-  uint64_t beg, end;
-  block_range(n, G, blockIdx.x, beg, end);
-  uint64_t gbase = partial[2 * blockIdx.x + 1];
   const uint32_t li = (uint32_t)(level - 1);
-  for (uint64_t tb = beg; tb < end; tb += TILE) {
-    const uint32_t tn = (uint32_t)((end - tb) < TILE ? (end - tb) : TILE);
-    load_tile(elem, n, tb, tn, sk);
-    const uint32_t l0 = threadIdx.x * EPT;
-    const uint64_t i0 = tb + l0;
-    uint32_t f[EPT];
-    uint64_t ng = 0;
+  const bool vec_out = ((uintptr_t)group_l & 15) == 0;
+  TileRun<K> r;
+  while (claim_tile(elem, n, ticket, &s_tile, s_edge, r)) {
+    // 4 flag bits per element: bit0 live (its mesh is still being built),
+    // bit1 group head, bit2 root
+    uint64_t f = 0;
+    uint32_t ng = 0;
+    {
+      uint32_t last_m = 0xFFFFFFFFu, done = 0;
+      K pk = r.prev;
+      bool hp = r.has_prev;
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      const uint32_t l = l0 + e;
-      f[e] = (l < tn) ? level_flags_v(sk[l], sk[l + 1], sk[l + 2], tb + l > 0, tb + l + 1 < n,
-                                      kl, state)
-                      : 0u;
-      ng += (f[e] >> 1) & 1u;
-    }
-    uint64_t tg;
-    uint64_t rg = block_excl_scan256(ng, tmp, tg) + gbase;
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      const uint64_t i = i0 + e;
-      if (i >= end) break;
-      const K key = sk[l0 + e + 1];
-      const uint32_t m = kl.mesh(key);
-      if (f[e] & 4u) {  // the only key of its mesh at this level: the root
-        state[m].depth = (uint32_t)level;
-        state[m].root_elem = (uint32_t)i;
-        group_l[i] = NONE;
-        LevelSpan& sp = spans[(uint64_t)m * n_levels + li];
-        sp.elem_first = (uint32_t)i;
-        sp.elem_end = (uint32_t)i + 1;
-        sp.group_first = sp.group_end = 0;
-        continue;
-      }
-      if (state[m].depth != 0) {  // depth-1 tree carried over from level 1
-        group_l[i] = NONE;
-        continue;
-      }
-      if (f[e] & 2u) {
-        elem_next[rg] = kl.parent(key);
-        first_child_next[rg] = (uint32_t)i;
-        ++rg;
-      }
-      group_l[i] = (uint32_t)(rg - 1);
-      const bool first = (i == 0) || kl.mesh(sk[l0 + e]) != m;
-      const bool last = (i + 1 == n) || kl.mesh(sk[l0 + e + 2]) != m;
-      LevelSpan& sp = spans[(uint64_t)m * n_levels + li];
-      if (first) {
-        sp.elem_first = (uint32_t)i;
-        sp.group_first = (uint32_t)(rg - 1);
-      }
-      if (last) {
-        sp.elem_end = (uint32_t)i + 1;
-        sp.group_end = (uint32_t)rg;
+      for (int e = 0; e < EPT; ++e) {
+        if ((uint32_t)e < r.cnt) {
+          const K cur = r.kk[e];
+          const uint32_t m = kl.mesh(cur);
+          if (m != last_m) {
+            done = state[m].depth;
+            last_m = m;
+          }
+          if (done == 0) {
+            const bool more = e + 1 < EPT && (uint32_t)(e + 1) < r.cnt;
+            const bool hn = more || r.has_next;
+            const K nk = more ? r.kk[(e + 1) % EPT] : r.next;
+            const uint32_t fl = level_flags_v(pk, cur, nk, hp, hn, kl) | 1u;
+            f |= (uint64_t)fl << (4 * e);
+            ng += (fl >> 1) & 1u;
+          }
+          pk = cur;
+          hp = true;
+        }
       }
     }
-    gbase += tg;
+    uint32_t tg;
+    const uint32_t exg = block_excl_scan256(ng, tmp, tg);
+    if (threadIdx.x == 0) {
+      const uint32_t pre = lookback_exclusive(status + 1, 2, r.tile, tg, epoch);
+      s_base = pre;
+      if (r.tb + r.tn == n) *count_next = (unsigned long long)pre + tg;
+    }
+    __syncthreads();
+    const uint32_t bg = s_base;
+    uint32_t gv[EPT];
+    {
+      uint32_t rg = exg;
+      K pk = r.prev;
+      bool hp = r.has_prev;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        gv[e] = NONE;
+        if ((uint32_t)e < r.cnt) {
+          const K key = r.kk[e];
+          const uint64_t i = r.i0 + e;
+          const uint32_t fl = (uint32_t)(f >> (4 * e)) & 15u;
+          const uint32_t m = kl.mesh(key);
+          if (fl & 4u) {  // the only key of its mesh at this level: the root
+            state[m].depth = (uint32_t)level;
+            state[m].root_elem = (uint32_t)i;
+            LevelSpan& sp = spans[(uint64_t)m * n_levels + li];
+            sp.elem_first = (uint32_t)i;
+            sp.elem_end = (uint32_t)i + 1;
+            sp.group_first = sp.group_end = 0;
+          } else if (fl & 1u) {
+            if (fl & 2u) {
+              s_key[rg] = kl.parent(key);
+              s_val[rg] = (uint32_t)i;
+              ++rg;
+            }
+            gv[e] = bg + rg - 1;
+            const bool more = e + 1 < EPT && (uint32_t)(e + 1) < r.cnt;
+            const bool hn = more || r.has_next;
+            const K nk = more ? r.kk[(e + 1) % EPT] : r.next;
+            LevelSpan& sp = spans[(uint64_t)m * n_levels + li];
+            if (!hp || kl.mesh(pk) != m) {
+              sp.elem_first = (uint32_t)i;
+              sp.group_first = bg + rg - 1;
+            }
+            if (!hn || kl.mesh(nk) != m) {
+              sp.elem_end = (uint32_t)i + 1;
+              sp.group_end = bg + rg;
+            }
+          }
+          pk = key;
+          hp = true;
+        }
+      }
+    }
+    if (vec_out && r.cnt == EPT) {
+      uint4* q = (uint4*)(group_l + r.i0);
+#pragma unroll
+      for (int v = 0; v < EPT / 4; ++v)
+        q[v] = make_uint4(gv[4 * v], gv[4 * v + 1], gv[4 * v + 2], gv[4 * v + 3]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < EPT; ++e)
+        if ((uint32_t)e < r.cnt) group_l[r.i0 + e] = gv[e];
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tg; i += BLOCK) {
+      elem_next[(uint64_t)bg + i] = s_key[i];
+      first_child_next[(uint64_t)bg + i] = s_val[i];
+    }
+    __syncthreads();
   }
+}
+
+uint32_t grid_tiles(uint64_t cap, uint32_t blocks) {
+  const uint64_t tiles = (cap + TILE - 1) / TILE;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, blocks));
 }
 
 }  // namespace
 
-size_t level_tmp_bytes(uint32_t blocks) { return (size_t)2 * blocks * sizeof(uint64_t); }
+size_t level_tmp_bytes(uint64_t cap) {
+  const uint64_t tiles = (std::max<uint64_t>(cap, 1) + TILE - 1) / TILE;
+  return 256 + 2 * tiles * sizeof(uint64_t);
+}
 
 template <typename K>
-void launch_level_first(const K* sorted, const unsigned long long* n_sorted, KeyLayout<K> kl,
-                        LevelArrays l1, LevelArrays l2, unsigned long long* counts,
-                        LevelSpan* spans1, uint32_t n_levels, uint32_t blocks, void* tmp,
-                        hipStream_t s) {
-  uint64_t* partial = (uint64_t*)tmp;
-  hipLaunchKernelGGL(k_first_count<K>, dim3(blocks), dim3(BLOCK), 0, s, sorted, n_sorted, kl,
-                     blocks, partial);
-  hipLaunchKernelGGL(k_pair_scan, dim3(1), dim3(BLOCK), 0, s, partial, blocks, counts,
-                     counts + 1);
-  hipLaunchKernelGGL(k_first_write<K>, dim3(blocks), dim3(BLOCK), 0, s, sorted, n_sorted, kl,
-                     blocks, (const uint64_t*)partial, (K*)l1.elem, l1.elem_group, (K*)l2.elem,
-                     l2.first_child, spans1, n_levels);
+void launch_level_first(const K* sorted, const unsigned long long* n_sorted, uint64_t cap,
+                        KeyLayout<K> kl, LevelArrays l1, LevelArrays l2,
+                        unsigned long long* counts, LevelSpan* spans1, uint32_t n_levels,
+                        uint32_t blocks, void* tmp, hipStream_t s) {
+  uint32_t* tickets = (uint32_t*)tmp;
+  uint64_t* status = (uint64_t*)((char*)tmp + 256);
+  const uint64_t tiles = (std::max<uint64_t>(cap, 1) + TILE - 1) / TILE;
+  const uint32_t cg = (uint32_t)std::min<uint64_t>(2048, (2 * tiles + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_level_clear, dim3(std::max<uint32_t>(cg, 1)), dim3(BLOCK), 0, s, status,
+                     tickets, n_sorted);
+  hipLaunchKernelGGL(k_first_pass<K>, dim3(grid_tiles(cap, blocks)), dim3(BLOCK), 0, s, sorted,
+                     n_sorted, kl, (K*)l1.elem, l1.elem_group, (K*)l2.elem, l2.first_child,
+                     spans1, n_levels, status, tickets, 1u, counts);
 }
 
 void launch_level_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint32_t n_levels,
@@ -339,35 +388,30 @@ void launch_level_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint
 
 template <typename K>
 void launch_level(int level, KeyLayout<K> kl, LevelArrays cur, LevelArrays next,
-                  unsigned long long* counts, LevelSpan* spans, uint32_t n_meshes,
-                  uint32_t n_levels, MeshTreeState* state, uint32_t blocks, void* tmp,
-                  hipStream_t s) {
-  (void)n_meshes;
-  uint64_t* partial = (uint64_t*)tmp;
+                  unsigned long long* counts, LevelSpan* spans, uint32_t n_levels,
+                  MeshTreeState* state, uint32_t blocks, void* tmp, hipStream_t s) {
+  uint32_t* tickets = (uint32_t*)tmp;
+  uint64_t* status = (uint64_t*)((char*)tmp + 256);
   // counts[level-1] = elements of this level; counts[level] = elements of the next
-  const unsigned long long* n_ptr = counts + (level - 1);
-  hipLaunchKernelGGL(k_level_count<K>, dim3(blocks), dim3(BLOCK), 0, s, (const K*)cur.elem,
-                     n_ptr, kl, (const MeshTreeState*)state, blocks, partial);
-  hipLaunchKernelGGL(k_pair_scan, dim3(1), dim3(BLOCK), 0, s, partial, blocks,
-                     (unsigned long long*)nullptr, counts + level);
-  hipLaunchKernelGGL(k_level_write<K>, dim3(blocks), dim3(BLOCK), 0, s, level,
-                     (const K*)cur.elem, n_ptr, kl, state, blocks, (const uint64_t*)partial,
-                     cur.elem_group, (K*)next.elem, next.first_child, spans, n_levels);
+  hipLaunchKernelGGL(k_level_pass<K>, dim3(grid_tiles(cur.cap, blocks)), dim3(BLOCK), 0, s,
+                     level, (const K*)cur.elem, counts + (level - 1), kl, state,
+                     cur.elem_group, (K*)next.elem, next.first_child, spans, n_levels, status,
+                     tickets + (level - 1), (uint32_t)level, counts + level);
 }
 
-template void launch_level_first<uint32_t>(const uint32_t*, const unsigned long long*,
+template void launch_level_first<uint32_t>(const uint32_t*, const unsigned long long*, uint64_t,
                                            KeyLayout<uint32_t>, LevelArrays, LevelArrays,
                                            unsigned long long*, LevelSpan*, uint32_t, uint32_t,
                                            void*, hipStream_t);
-template void launch_level_first<uint64_t>(const uint64_t*, const unsigned long long*,
+template void launch_level_first<uint64_t>(const uint64_t*, const unsigned long long*, uint64_t,
                                            KeyLayout<uint64_t>, LevelArrays, LevelArrays,
                                            unsigned long long*, LevelSpan*, uint32_t, uint32_t,
                                            void*, hipStream_t);
 template void launch_level<uint32_t>(int, KeyLayout<uint32_t>, LevelArrays, LevelArrays,
-                                     unsigned long long*, LevelSpan*, uint32_t, uint32_t,
-                                     MeshTreeState*, uint32_t, void*, hipStream_t);
+                                     unsigned long long*, LevelSpan*, uint32_t, MeshTreeState*,
+                                     uint32_t, void*, hipStream_t);
 template void launch_level<uint64_t>(int, KeyLayout<uint64_t>, LevelArrays, LevelArrays,
-                                     unsigned long long*, LevelSpan*, uint32_t, uint32_t,
-                                     MeshTreeState*, uint32_t, void*, hipStream_t);
+                                     unsigned long long*, LevelSpan*, uint32_t, MeshTreeState*,
+                                     uint32_t, void*, hipStream_t);
 
 }  // namespace svo

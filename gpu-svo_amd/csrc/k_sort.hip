@@ -35,23 +35,11 @@ constexpr int MAX_PASSES = 8;
 constexpr int KPT = 16;                       // keys per lane per tile
 constexpr int TILE = WAVES * 64 * KPT;        // 4096
 constexpr int UP_KEYS = 16;                   // consecutive keys per upsweep thread
-constexpr int LOOKBACK = 16;                  // status words loaded per look-back step
-
-// Status word of (tile, digit): bits 62-63 flag (1 = tile count only,
-// 2 = inclusive prefix), bits 32-61 pass epoch (pass index + 1), bits 0-31
-// value.  The words of the used tiles are zeroed before the first pass, so a
-// word whose epoch is not the current pass's has not been written yet.
-constexpr uint64_t ST_AGG = 1ull << 62, ST_INCL = 2ull << 62;
-constexpr uint64_t ST_FLAGS = 3ull << 62;
 
 // Digit of the pass at `shift` covers bits [shift, min(shift + 8, bits)).
 __host__ __device__ inline uint32_t digit_mask(int bits, int shift) {
   const int w = bits - shift < 8 ? bits - shift : 8;
   return (1u << w) - 1u;
-}
-
-__device__ inline uint64_t st_word(uint64_t flag, uint32_t epoch, uint32_t v) {
-  return flag | ((uint64_t)epoch << 32) | v;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_sort_clear(uint64_t* status, uint32_t* hist,
@@ -65,24 +53,6 @@ __global__ __launch_bounds__(BLOCK) void k_sort_clear(uint64_t* status, uint32_t
   if (blockIdx.x == 0) {
     for (int i = threadIdx.x; i < MAX_PASSES * RADIX; i += BLOCK) hist[i] = 0;
     if (threadIdx.x < MAX_PASSES) tickets[threadIdx.x] = 0;
-  }
-}
-
-template <typename K>
-__device__ inline void load_run(const K* p, bool vec, uint32_t cnt, K (&kk)[UP_KEYS]) {
-  if (vec && cnt == UP_KEYS) {
-    const uint4* q = (const uint4*)p;
-    constexpr int PER = 16 / sizeof(K);
-#pragma unroll
-    for (int v = 0; v < UP_KEYS / PER; ++v) {
-      const uint4 x = q[v];
-      const K* e = (const K*)&x;
-#pragma unroll
-      for (int j = 0; j < PER; ++j) kk[v * PER + j] = e[j];
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < UP_KEYS; ++j) kk[j] = (uint32_t)j < cnt ? p[j] : (K)0;
   }
 }
 
@@ -105,7 +75,7 @@ __global__ __launch_bounds__(BLOCK) void k_upsweep(const K* in, const unsigned l
     const uint64_t b = c * UP_KEYS;
     const uint32_t cnt = (uint32_t)((n - b) < UP_KEYS ? (n - b) : UP_KEYS);
     K kk[UP_KEYS];
-    load_run(in + b, vec, cnt, kk);
+    load_consecutive(in + b, vec, cnt, kk);
     for (int p = 0; p < passes; ++p) {
       const int sh = 8 * p;
       const uint32_t dm = digit_mask(bits, sh);
@@ -171,31 +141,38 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const K* in, K* out,
     const uint64_t tb = tile * TILE;
     if (tb >= n) return;  // uniform over the block
     const uint32_t tn = (uint32_t)((n - tb) < TILE ? (n - tb) : TILE);
+    // loads first, all in flight together (indices clamped, so unconditional)
     K keys[KPT];
-    uint32_t rank[KPT];
+#pragma unroll
+    for (int it = 0; it < KPT; ++it) {
+      const uint32_t pos = w * (KPT * 64) + it * 64 + lane;
+      const K v = in[tb + (pos < tn ? pos : tn - 1)];
+      keys[it] = pos < tn ? v : (K)0;
+    }
+    // stable rank within the wave: match masks per 64 keys; the group leader
+    // reserves the group's slots in the wave's digit counter (LDS atomics
+    // return the old counts; the 16 reservations are waited on together)
+    uint32_t rank[KPT], lead[KPT], base[KPT];
 #pragma unroll
     for (int it = 0; it < KPT; ++it) {
       const uint32_t pos = w * (KPT * 64) + it * 64 + lane;
       const bool valid = pos < tn;
-      const K key = valid ? in[tb + pos] : (K)0;
-      const uint32_t d = (uint32_t)(key >> shift) & dm;
+      const uint32_t d = (uint32_t)(keys[it] >> shift) & dm;
       uint64_t eq = __ballot(valid);
 #pragma unroll
       for (int bit = 0; bit < 8; ++bit) {
-        const bool b = (d >> bit) & 1u;
-        const uint64_t m = __ballot(b);
-        eq &= b ? m : ~m;
+        // lanes whose bit equals this lane's: ~(ballot ^ (bit ? ~0 : 0))
+        const uint32_t sx = (uint32_t)((int32_t)(d << (31 - bit)) >> 31);
+        const uint64_t m = __ballot(sx != 0);
+        eq &= ~(m ^ (((uint64_t)sx << 32) | sx));
       }
-      const uint32_t leader = valid ? (uint32_t)__builtin_ctzll(eq) : 0u;
-      uint32_t old = 0;
-      if (valid && lane == leader) {
-        old = wcnt[w][d];
-        wcnt[w][d] = old + (uint32_t)__popcll(eq);
-      }
-      old = __shfl(old, (int)leader);
-      rank[it] = old + lanes_below(eq);
-      keys[it] = key;
+      lead[it] = valid ? (uint32_t)__builtin_ctzll(eq) : 0u;
+      rank[it] = lanes_below(eq);
+      base[it] = 0;
+      if (valid && lane == lead[it]) base[it] = atomicAdd(&wcnt[w][d], (uint32_t)__popcll(eq));
     }
+#pragma unroll
+    for (int it = 0; it < KPT; ++it) rank[it] += __shfl(base[it], (int)lead[it]);
     __syncthreads();
     {
       const uint32_t d = threadIdx.x;
@@ -206,49 +183,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const K* in, K* out,
         wcnt[k][d] = cnt;
         cnt += c;
       }
-      uint64_t* st = status + tile * RADIX + d;
-      uint32_t prefix = 0;
-      if (tile == 0) {
-        __hip_atomic_store(st, st_word(ST_INCL, epoch, cnt), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      } else {
-        __hip_atomic_store(st, st_word(ST_AGG, epoch, cnt), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        // Look back over a window of LOOKBACK earlier tiles at a time: the
-        // loads are independent, so one memory latency covers the window.
-        // Words are consumed nearest first until an inclusive one; a word
-        // not yet published ends the window and is loaded again.
-        int64_t t = (int64_t)tile - 1;
-        while (true) {
-          uint64_t wv[LOOKBACK];
-#pragma unroll
-          for (int j = 0; j < LOOKBACK; ++j)
-            wv[j] = t - j >= 0 ? __hip_atomic_load(status + (uint64_t)(t - j) * RADIX + d,
-                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                               : 0ull;
-          int used = 0;
-          bool stop = false, found = false;
-#pragma unroll
-          for (int j = 0; j < LOOKBACK; ++j) {
-            if (!stop) {
-              const uint64_t x = wv[j];
-              if ((x & ST_FLAGS) == 0 || (uint32_t)((x >> 32) & 0x3FFFFFFFu) != epoch) {
-                stop = true;
-              } else {
-                prefix += (uint32_t)x;
-                ++used;
-                if ((x & ST_FLAGS) == ST_INCL) stop = found = true;
-              }
-            }
-          }
-          if (found) break;
-          // tile 0 always publishes an inclusive word, so t - used stays >= 0
-          t -= used;
-          if (used < LOOKBACK) __builtin_amdgcn_s_sleep(1);
-        }
-        __hip_atomic_store(st, st_word(ST_INCL, epoch, prefix + cnt), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-      }
+      const uint32_t prefix = lookback_exclusive(status + d, RADIX, tile, cnt, epoch);
       dbase[d] = d_off + prefix;
       uint32_t total;
       tstart[d] = block_excl_scan256(cnt, tmp, total);
@@ -263,10 +198,20 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const K* in, K* out,
       }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < tn; i += BLOCK) {
-      const K key = sorted[i];
-      const uint32_t d = (uint32_t)(key >> shift) & dm;
-      out[(uint64_t)dbase[d] + (i - tstart[d])] = key;
+    // write out in tile order: consecutive threads store consecutive keys of
+    // a digit run
+    K sk[KPT];
+#pragma unroll
+    for (int it = 0; it < KPT; ++it) {
+      const uint32_t i = it * BLOCK + threadIdx.x;
+      sk[it] = sorted[i < tn ? i : 0];
+    }
+#pragma unroll
+    for (int it = 0; it < KPT; ++it) {
+      const uint32_t i = it * BLOCK + threadIdx.x;
+      const uint32_t d = (uint32_t)(sk[it] >> shift) & dm;
+      const uint64_t o = (uint64_t)dbase[d] + (i - tstart[d]);
+      if (i < tn) out[o] = sk[it];
     }
     __syncthreads();
   }

@@ -442,7 +442,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, max_blocks));
   };
   SVO_TRY(W.get("sort_tmp", radix_sort_tmp_bytes(key_cap), &sort_tmp));
-  SVO_TRY(W.get("lvl_tmp", level_tmp_bytes(C.tuning.scan_blocks), &lvl_tmp));
+  SVO_TRY(W.get("lvl_tmp", level_tmp_bytes(key_cap), &lvl_tmp));
   // level capacities: cap_1 = box cells (>= hits >= unique voxels);
   // cap_l = min(cap_{l-1}, number of level-l cells over the batch)
   std::vector<uint64_t> cap(NL + 2, 0);
@@ -499,13 +499,13 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   launch_radix_sort<K>(keys_a, keys_b, d_cnt + 2, key_cap, plan.cb + plan.mb, C.tuning.sort_blocks,
                        sort_tmp, &sorted, s);
   timer.mark("sort");
-  launch_level_first<K>(sorted, d_cnt + 2, kl, lv[1], lv[2], d_counts, d_spans, NL,
-                        grid_for(key_cap, 8192, C.tuning.scan_blocks), lvl_tmp, s);
+  launch_level_first<K>(sorted, d_cnt + 2, key_cap, kl, lv[1], lv[2], d_counts, d_spans, NL,
+                        C.tuning.scan_blocks, lvl_tmp, s);
   launch_level_first_finalize(d_spans, n, NL, d_state, s);
   timer.mark("unique_level1");
   for (uint32_t l = 2; l <= NL; ++l)
-    launch_level<K>((int)l, kl, lv[l], lv[l + 1], d_counts, d_spans, n, NL, d_state,
-                    grid_for(cap[l], 8192, C.tuning.scan_blocks), lvl_tmp, s);
+    launch_level<K>((int)l, kl, lv[l], lv[l + 1], d_counts, d_spans, NL, d_state,
+                    C.tuning.scan_blocks, lvl_tmp, s);
   timer.mark("levels");
 
   // ---- readback of the level phase

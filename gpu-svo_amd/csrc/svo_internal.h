@@ -54,12 +54,14 @@ struct LevelArrays {
 };
 // Level 1 from the sorted hit keys: unique codes, their groups, and the
 // level-2 element array (parents).  Counts go to counts[0] (level 1 elements)
-// and counts[1] (level 2 elements).
+// and counts[1] (level 2 elements).  n_sorted <= cap; `blocks` bounds the
+// persistent grid; `tmp` needs level_tmp_bytes(cap) bytes and is shared with
+// the later levels of the same pass (launch_level_first prepares it).
 template <typename K>
-void launch_level_first(const K* sorted, const unsigned long long* n_sorted, KeyLayout<K> kl,
-                        LevelArrays l1, LevelArrays l2, unsigned long long* counts,
-                        LevelSpan* spans1, uint32_t n_levels, uint32_t blocks, void* tmp,
-                        hipStream_t s);
+void launch_level_first(const K* sorted, const unsigned long long* n_sorted, uint64_t cap,
+                        KeyLayout<K> kl, LevelArrays l1, LevelArrays l2,
+                        unsigned long long* counts, LevelSpan* spans1, uint32_t n_levels,
+                        uint32_t blocks, void* tmp, hipStream_t s);
 // Marks meshes whose level-1 set holds a single voxel as depth-1 trees.
 void launch_level_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint32_t n_levels,
                                  MeshTreeState* state, hipStream_t s);
@@ -67,10 +69,9 @@ void launch_level_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint
 // the others feed level l+1.
 template <typename K>
 void launch_level(int level, KeyLayout<K> kl, LevelArrays cur, LevelArrays next,
-                  unsigned long long* counts, LevelSpan* spans, uint32_t n_meshes,
-                  uint32_t n_levels, MeshTreeState* state, uint32_t blocks, void* tmp,
-                  hipStream_t s);
-size_t level_tmp_bytes(uint32_t blocks);
+                  unsigned long long* counts, LevelSpan* spans, uint32_t n_levels,
+                  MeshTreeState* state, uint32_t blocks, void* tmp, hipStream_t s);
+size_t level_tmp_bytes(uint64_t cap);
 
 // ------------------------------------------------------------ k_attrs.hip
 // Per mesh node layout computed on the host after the level phase.

@@ -48,4 +48,88 @@ struct U64x3 {
   uint64_t a, b, c;
 };
 
+// Loads cnt <= N consecutive keys p[0 .. cnt) into kk (the rest zero); with
+// vec set (p 16-byte aligned) a full run is loaded as 16-byte vectors.
+template <typename K, int N>
+__device__ inline void load_consecutive(const K* p, bool vec, uint32_t cnt, K (&kk)[N]) {
+  static_assert((N * sizeof(K)) % 16 == 0, "runs are whole 16-byte vectors");
+  if (vec && cnt == (uint32_t)N) {
+    const uint4* q = (const uint4*)p;
+    constexpr int PER = 16 / sizeof(K);
+#pragma unroll
+    for (int v = 0; v < N / PER; ++v) {
+      const uint4 x = q[v];
+      const K* e = (const K*)&x;
+#pragma unroll
+      for (int j = 0; j < PER; ++j) kk[v * PER + j] = e[j];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < N; ++j) kk[j] = (uint32_t)j < cnt ? p[j] : (K)0;
+  }
+}
+
+// ------------------------------------------------------ decoupled look-back
+// Single-pass scans over tiles that workgroups claim in order with an atomic
+// ticket.  Tile t publishes its count in a 64-bit status word -- bits 62-63
+// flag (1 = the tile's own count, 2 = inclusive prefix), bits 32-61 epoch,
+// bits 0-31 value -- and sums the words of earlier tiles, nearest first,
+// until it meets an inclusive one.  Words whose epoch differs from the
+// caller's have not been written by this pass: the words of the tiles in use
+// are zeroed once and every pass over them uses its own nonzero epoch.
+// Every earlier tile must be held by a running workgroup that publishes its
+// own count without waiting on a later tile; then every wait ends.
+constexpr uint64_t LB_AGG = 1ull << 62, LB_INCL = 2ull << 62, LB_FLAGS = 3ull << 62;
+constexpr int LB_WINDOW = 16;  // words loaded per step: one latency covers them
+
+__device__ inline uint64_t lb_word(uint64_t flag, uint32_t epoch, uint32_t v) {
+  return flag | ((uint64_t)(epoch & 0x3FFFFFFFu) << 32) | v;
+}
+
+// Publishes `count` of tile `tile` at words[tile * stride] and returns the
+// sum of the counts of tiles 0 .. tile-1.
+__device__ inline uint32_t lookback_exclusive(uint64_t* words, uint64_t stride, uint64_t tile,
+                                              uint32_t count, uint32_t epoch) {
+  epoch &= 0x3FFFFFFFu;
+  uint64_t* own = words + tile * stride;
+  if (tile == 0) {
+    __hip_atomic_store(own, lb_word(LB_INCL, epoch, count), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  __hip_atomic_store(own, lb_word(LB_AGG, epoch, count), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t prefix = 0;
+  int64_t t = (int64_t)tile - 1;
+  while (true) {
+    uint64_t wv[LB_WINDOW];
+#pragma unroll
+    for (int j = 0; j < LB_WINDOW; ++j)
+      wv[j] = t - j >= 0 ? __hip_atomic_load(words + (uint64_t)(t - j) * stride,
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         : 0ull;
+    int used = 0;
+    bool stop = false, found = false;
+#pragma unroll
+    for (int j = 0; j < LB_WINDOW; ++j) {
+      if (!stop) {
+        const uint64_t x = wv[j];
+        if ((x & LB_FLAGS) == 0 || (uint32_t)((x >> 32) & 0x3FFFFFFFu) != epoch) {
+          stop = true;  // not yet published: load it again
+        } else {
+          prefix += (uint32_t)x;
+          ++used;
+          if ((x & LB_FLAGS) == LB_INCL) stop = found = true;
+        }
+      }
+    }
+    if (found) break;
+    t -= used;  // tile 0 publishes an inclusive word, so t stays >= 0
+    if (used < LB_WINDOW) __builtin_amdgcn_s_sleep(1);
+  }
+  __hip_atomic_store(own, lb_word(LB_INCL, epoch, prefix + count), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  return prefix;
+}
+
 }  // namespace svo
