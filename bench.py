@@ -41,12 +41,17 @@ def stage_bytes(stats, name):
     """Algorithmic HBM bytes of one pipeline stage (read + write, each once)."""
     kb = stats["key_bytes"]
     H, U, N = stats["n_hits"], stats["n_voxels"], stats["n_nodes"]
+    B = stats["bitmap_bytes"]
     if name == "sort":
         return 2 * kb * H * stats["sort_passes"]
+    if name == "bitmap_clear":
+        return B
     if name == "voxelize":
-        return 36 * stats["n_tris"] + kb * H
+        # triangles in; raw hits out (keys, or one bitmap word update each)
+        return 36 * stats["n_tris"] + (4 if B else kb) * H
     if name == "unique_level1":
-        return kb * H + (kb + 4) * U + 8 * U
+        # sorted hits (or the bitmap) in; unique keys + groups out
+        return (B if B else kb * H) + (kb + 4) * U + 8 * U
     if name == "levels":
         return (kb + 4) * stats["level_elems"]
     if name == "neighbor_fill":
@@ -60,6 +65,14 @@ def stage_bytes(stats, name):
     return 0
 
 
+def dedup_gbs(stats, stages):
+    """HBM rate of the de-duplication stages (bitmap clear or sort, and the
+    compaction into level 1): algorithmic bytes over their device time."""
+    names = [k for k in ("bitmap_clear", "sort", "unique_level1") if k in stages]
+    t = sum(stages[k] for k in names)
+    return round(sum(stage_bytes(stats, k) for k in names) / (t * 1e-3) / 1e9, 1) if t else None
+
+
 def pass_stats(ctx):
     import ctypes as C
     import svo_amd
@@ -69,7 +82,8 @@ def pass_stats(ctx):
         _fields_ = [(k, C.c_uint64) for k in (
             "n_meshes", "n_tris", "box_cells", "n_hits", "n_voxels", "level_elems", "n_nodes",
             "n_vertices", "n_edges", "n_faces", "out_bytes")] + \
-            [(k, C.c_uint32) for k in ("key_bytes", "sort_bits", "sort_passes", "n_levels")]
+            [(k, C.c_uint32) for k in ("key_bytes", "sort_bits", "sort_passes", "n_levels")] + \
+            [("bitmap_bytes", C.c_uint64)]
     fn = L.svo_ctx_last_pass_stats
     fn.argtypes = [C.c_void_p, C.POINTER(PassStats)]
     ps = PassStats()
@@ -177,7 +191,6 @@ def run_batch_workload(args):
     dom = max(dev, key=dev.get)
     dom_bytes = stage_bytes(stats, dom)
     achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9 if stages[dom] > 0 else 0.0
-    sort_gbs = stage_bytes(stats, "sort") / (stages["sort"] * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(256 / (ms_batch * 1e-3), 2),
@@ -206,7 +219,8 @@ def run_batch_workload(args):
                   "vertices": stats["n_vertices"], "edges": stats["n_edges"],
                   "out_gb": round(stats["out_bytes"] / 1e9, 3)},
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-        "sort_gbs": round(sort_gbs, 1),
+        "dedup_gbs": dedup_gbs(stats, stages),
+        "dedup_path": "bitmap" if stats["bitmap_bytes"] else "sort",
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -259,7 +273,6 @@ def run_single_workload(args, name, P, T, res, key_bits, flags=0, desc=""):
     stats = pass_stats(ctx)
     ms = (t1 - t0) * 1e3 / args.steps
     stages = {k: v / args.steps for k, v in stage_acc.items()}
-    sort_c = stage_bytes(stats, "sort") + stage_bytes(stats, "unique_level1")
     out = {
         "metric": METRIC, "value": round(1e3 / ms, 3), "unit": "meshes/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
@@ -271,8 +284,8 @@ def run_single_workload(args, name, P, T, res, key_bits, flags=0, desc=""):
         "sizes": {"voxels": nv, "nodes": nn, "depth": depth, "hits": stats["n_hits"],
                   "box_cells": stats["box_cells"], "out_gb": round(stats["out_bytes"] / 1e9, 3)},
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
-        "sort_compaction_gbs": round(sort_c / ((stages["sort"] + stages["unique_level1"]) * 1e-3)
-                                     / 1e9, 1),
+        "dedup_gbs": dedup_gbs(stats, stages),
+        "dedup_path": "bitmap" if stats["bitmap_bytes"] else "sort",
         "roofline": _roofline(stats, stages),
     }
     dm.free()

@@ -109,6 +109,58 @@ int svo_ctx_set_timing(svo_ctx* ctx, int enable) {
   return SVO_OK;
 }
 
+namespace {
+// name -> (field, min, max)
+struct OptionRef {
+  const char* name;
+  int64_t lo, hi;
+};
+const OptionRef kOptions[] = {
+    {"small_max_cells", 1, 1 << 20},
+    {"sort_blocks", 1, 1 << 16},
+    {"scan_blocks", 1, 1 << 16},
+    {"bitmap_max_bytes", 0, INT64_MAX},
+    {"bitmap_bytes_per_cell", 0, 1 << 20},
+    {"dedup_path", 0, 2},
+};
+}  // namespace
+
+int svo_ctx_set_option(svo_ctx* ctx, const char* name, int64_t value) {
+  if (!ctx || !name) return SVO_E_INVALID;
+  Context& C = ctx->impl;
+  std::lock_guard<std::mutex> lk(C.mu);
+  Tuning& t = C.tuning;
+  for (const OptionRef& o : kOptions) {
+    if (std::strcmp(o.name, name) != 0) continue;
+    if (value < o.lo || value > o.hi) return SVO_E_INVALID;
+    const std::string n(name);
+    if (n == "small_max_cells") t.small_max_cells = (uint32_t)value;
+    else if (n == "sort_blocks") t.sort_blocks = (uint32_t)value;
+    else if (n == "scan_blocks") t.scan_blocks = (uint32_t)value;
+    else if (n == "bitmap_max_bytes") t.bitmap_max_bytes = (uint64_t)value;
+    else if (n == "bitmap_bytes_per_cell") t.bitmap_bytes_per_cell = (uint32_t)value;
+    else t.force_path = (int)value;
+    return SVO_OK;
+  }
+  return SVO_E_INVALID;
+}
+
+int svo_ctx_get_option(svo_ctx* ctx, const char* name, int64_t* value) {
+  if (!ctx || !name || !value) return SVO_E_INVALID;
+  Context& C = ctx->impl;
+  std::lock_guard<std::mutex> lk(C.mu);
+  const Tuning& t = C.tuning;
+  const std::string n(name);
+  if (n == "small_max_cells") *value = t.small_max_cells;
+  else if (n == "sort_blocks") *value = t.sort_blocks;
+  else if (n == "scan_blocks") *value = t.scan_blocks;
+  else if (n == "bitmap_max_bytes") *value = (int64_t)t.bitmap_max_bytes;
+  else if (n == "bitmap_bytes_per_cell") *value = t.bitmap_bytes_per_cell;
+  else if (n == "dedup_path") *value = t.force_path;
+  else return SVO_E_INVALID;
+  return SVO_OK;
+}
+
 int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap, int* n) {
   if (!ctx || !n) return SVO_E_INVALID;
   Context& C = ctx->impl;

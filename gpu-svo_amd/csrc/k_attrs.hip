@@ -24,6 +24,8 @@
 // its predecessor is dropped (uniqueVert / uniqueEdge).  INTENDED: a vertex,
 // edge or face is kept by the same-level sharing node with the smallest
 // index.  Count -> scan -> write, node-major.
+#include <algorithm>
+
 #include "svo_internal.h"
 #include "svo_scan.h"
 
@@ -214,12 +216,16 @@ __global__ void k_copy_voxels(KeyLayout<K> kl, const K* elem1, uint64_t n, void*
   else ((uint32_t*)out)[i] = (uint32_t)c;
 }
 
+// Grid-stride (n may exceed what one launch can index: 108 bytes of
+// neighbors per node are 2^32 words at 160M nodes).  p is 16-byte aligned.
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
-  const uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
-  if (i + 3 < n) {
-    *reinterpret_cast<uint4*>(p + i) = make_uint4(v, v, v, v);
-  } else {
-    for (uint64_t j = i; j < n; ++j) p[j] = v;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 4;
+  for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4; i < n; i += stride) {
+    if (i + 3 < n) {
+      store_stream(reinterpret_cast<uint4*>(p + i), make_uint4(v, v, v, v));
+    } else {
+      for (uint64_t j = i; j < n; ++j) p[j] = v;
+    }
   }
 }
 
@@ -567,7 +573,7 @@ constexpr int REF_NODES = 64;
 __device__ inline void stream_out(const uint32_t* src, uint32_t n_dw, uint32_t* dst) {
   const uint32_t nq = n_dw >> 2;
   for (uint32_t q = threadIdx.x; q < nq; q += BLOCK)
-    reinterpret_cast<uint4*>(dst)[q] = reinterpret_cast<const uint4*>(src)[q];
+    store_stream(reinterpret_cast<uint4*>(dst) + q, reinterpret_cast<const uint4*>(src)[q]);
   for (uint32_t d = (nq << 2) + threadIdx.x; d < n_dw; d += BLOCK) dst[d] = src[d];
 }
 
@@ -703,7 +709,8 @@ void launch_copy_voxels(KeyLayout<K> kl, const void* elem1, uint64_t n, void* ou
 
 void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for((n + 3) / 4)), dim3(BLOCK), 0, s, p, value, n);
+  const uint64_t blocks = std::min<uint64_t>(grid_for((n + 3) / 4), 1u << 20);
+  hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)blocks), dim3(BLOCK), 0, s, p, value, n);
 }
 
 template <typename K>

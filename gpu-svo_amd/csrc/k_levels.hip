@@ -217,6 +217,272 @@ __global__ __launch_bounds__(BLOCK) void k_first_pass(
   }
 }
 
+// ---------------------------------------------------------------- level 1 from the bitmap
+// The occupancy bitmap holds, per mesh, one bit per cell in Morton order, so
+// its set bits in word order are the sorted unique keys and the nonzero bytes
+// are the parent groups (the eight children of parent p are codes 8p..8p+7,
+// one byte).  count -> scan of tile counts -> emit, each over tiles of 2048
+// words.
+constexpr int BM_WPT = 8;                 // words per thread (two 16-byte loads)
+constexpr int BM_TILE = BLOCK * BM_WPT;   // 2048 words
+
+__device__ inline uint32_t nonzero_bytes(uint32_t w) {
+  const uint32_t t = (w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;  // bit 7 of a byte: low 7 bits nonzero
+  return __popc((t | w) & 0x80808080u);
+}
+
+__device__ inline void load_words(const uint32_t* bits, uint64_t n_words, uint64_t w0,
+                                  uint32_t (&w)[BM_WPT]) {
+  if (w0 + BM_WPT <= n_words) {
+    const uint4 a = ((const uint4*)(bits + w0))[0], b = ((const uint4*)(bits + w0))[1];
+    w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
+    w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+  } else {
+#pragma unroll
+    for (int j = 0; j < BM_WPT; ++j) w[j] = w0 + j < n_words ? bits[w0 + j] : 0u;
+  }
+}
+
+constexpr int SC_PER = 16;                  // tile counts per thread in the scan
+constexpr int SC_CHUNK = BLOCK * SC_PER;     // 4096 tiles per scan workgroup
+
+__global__ __launch_bounds__(BLOCK) void k_bm_count(const uint32_t* bits, uint64_t n_words,
+                                                    uint32_t* tile_u, uint32_t* tile_g,
+                                                    uint64_t* scan_status, uint32_t n_status) {
+  __shared__ uint64_t tmp[4];
+  // the tile-count scan's look-back words and ticket start at zero
+  if (blockIdx.x == 0)
+    for (uint32_t i = threadIdx.x; i < n_status; i += BLOCK) scan_status[i] = 0;
+  const uint64_t w0 = (uint64_t)blockIdx.x * BM_TILE + threadIdx.x * BM_WPT;
+  uint32_t w[BM_WPT];
+  load_words(bits, n_words, w0, w);
+  uint32_t u = 0, g = 0;
+#pragma unroll
+  for (int j = 0; j < BM_WPT; ++j) {
+    u += __popc(w[j]);
+    g += nonzero_bytes(w[j]);
+  }
+  uint64_t tot;
+  block_excl_scan256(((uint64_t)g << 32) | u, tmp, tot);
+  if (threadIdx.x == 0) {
+    tile_u[blockIdx.x] = (uint32_t)tot;
+    tile_g[blockIdx.x] = (uint32_t)(tot >> 32);
+  }
+}
+
+// Exclusive scan of the tile counts in place, 4096 tiles per workgroup, chunk
+// offsets by decoupled look-back (chunks claimed by ticket); totals to
+// counts[0] (unique keys = level-1 elements) and counts[1] (groups = level-2
+// elements).
+__global__ __launch_bounds__(BLOCK) void k_bm_scan(uint32_t* tile_u, uint32_t* tile_g,
+                                                   uint32_t tiles, uint64_t* scan_status,
+                                                   unsigned long long* counts) {
+  __shared__ uint64_t tmp[4];
+  __shared__ uint32_t s_chunk, s_base[2];
+  uint32_t* ticket = (uint32_t*)scan_status;  // word 0; chunk c uses words 2 + 2c, 3 + 2c
+  if (threadIdx.x == 0) s_chunk = atomicAdd(ticket, 1u);
+  __syncthreads();
+  const uint32_t chunk = s_chunk;
+  const uint32_t i0 = chunk * SC_CHUNK + threadIdx.x * SC_PER;
+  uint32_t a[SC_PER], b[SC_PER];
+#pragma unroll
+  for (int j = 0; j < SC_PER; ++j) {
+    const uint32_t i = i0 + j;
+    const uint32_t c = i < tiles ? i : tiles - 1;
+    a[j] = tile_u[c];
+    b[j] = tile_g[c];
+  }
+  uint32_t su = 0, sg = 0;
+#pragma unroll
+  for (int j = 0; j < SC_PER; ++j)
+    if (i0 + j < tiles) {
+      su += a[j];
+      sg += b[j];
+    }
+  uint64_t tot;
+  const uint64_t ex = block_excl_scan256(((uint64_t)sg << 32) | su, tmp, tot);
+  if (threadIdx.x < 2) {
+    const uint32_t c = threadIdx.x == 0 ? (uint32_t)tot : (uint32_t)(tot >> 32);
+    const uint32_t pre = lookback_exclusive(scan_status + 2 + threadIdx.x, 2, chunk, c, 1u);
+    s_base[threadIdx.x] = pre;
+    if ((uint64_t)(chunk + 1) * SC_CHUNK >= tiles) counts[threadIdx.x] = (unsigned long long)pre + c;
+  }
+  __syncthreads();
+  uint32_t ru = s_base[0] + (uint32_t)ex, rg = s_base[1] + (uint32_t)(ex >> 32);
+#pragma unroll
+  for (int j = 0; j < SC_PER; ++j) {
+    const uint32_t i = i0 + j;
+    if (i < tiles) {
+      tile_u[i] = ru;
+      tile_g[i] = rg;
+      ru += a[j];
+      rg += b[j];
+    }
+  }
+}
+
+__device__ inline uint32_t mesh_of_word(const uint64_t* bm_off, uint32_t n_meshes, uint64_t gw) {
+  uint32_t lo = 0, hi = n_meshes;  // bm_off[lo] <= gw < bm_off[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (bm_off[mid] <= gw) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Position of the k-th (0-based) set bit of w; w has more than k set bits.
+__device__ inline uint32_t select_bit(uint32_t w, uint32_t k) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (int half = 16; half >= 1; half >>= 1) {
+    const uint32_t lo = __popc(w & ((1u << half) - 1u));
+    if (k >= lo) {
+      k -= lo;
+      w >>= half;
+      pos += half;
+    }
+  }
+  return pos;
+}
+
+// Nonzero bytes of w among bytes 0 .. y.
+__device__ inline uint32_t nonzero_bytes_upto(uint32_t w, uint32_t y) {
+  return nonzero_bytes(y >= 3 ? w : (w & ((1u << (8 * y + 8)) - 1u)));
+}
+
+// Index of the last entry of the sorted array a[0 .. n) that is <= v
+// (a[0] = 0 <= v), by binary search in LDS.
+__device__ inline uint32_t last_le(const uint32_t* a, uint32_t n, uint32_t v) {
+  uint32_t lo = 0, hi = n;  // a[lo] <= v < a[hi] (a[n] taken as infinite)
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] <= v) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// One tile of 2048 words.  The tile's words and the exclusive prefixes of
+// their set bits and nonzero bytes go to LDS; then every output is produced
+// by one thread, in output order, so the stores are contiguous: unique key o
+// is the k-th set bit of the word whose bit prefix holds o, group g the k-th
+// nonzero byte of the word whose byte prefix holds g.
+template <typename K>
+__global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_t n_words,
+                                                   const uint64_t* bm_off, uint32_t n_meshes,
+                                                   KeyLayout<K> kl, const uint32_t* tile_u,
+                                                   const uint32_t* tile_g, K* elem1,
+                                                   uint32_t* group1, K* elem2,
+                                                   uint32_t* first_child2) {
+  __shared__ uint64_t tmp[4];
+  __shared__ uint32_t s_word[BM_TILE], s_upre[BM_TILE], s_gpre[BM_TILE];
+  __shared__ uint32_t s_mesh;
+  const uint64_t tw0 = (uint64_t)blockIdx.x * BM_TILE;
+  const uint32_t l0 = threadIdx.x * BM_WPT;
+  if (threadIdx.x == 0) s_mesh = mesh_of_word(bm_off, n_meshes, tw0);
+  uint32_t w[BM_WPT];
+  load_words(bits, n_words, tw0 + l0, w);
+  uint32_t u = 0, g = 0;
+#pragma unroll
+  for (int j = 0; j < BM_WPT; ++j) {
+    u += __popc(w[j]);
+    g += nonzero_bytes(w[j]);
+  }
+  uint64_t tot;
+  const uint64_t ex = block_excl_scan256(((uint64_t)g << 32) | u, tmp, tot);
+  const uint32_t tu = (uint32_t)tot, tg = (uint32_t)(tot >> 32);
+  {
+    uint32_t pu = (uint32_t)ex, pg = (uint32_t)(ex >> 32);
+#pragma unroll
+    for (int j = 0; j < BM_WPT; ++j) {
+      s_word[l0 + j] = w[j];
+      s_upre[l0 + j] = pu;
+      s_gpre[l0 + j] = pg;
+      pu += __popc(w[j]);
+      pg += nonzero_bytes(w[j]);
+    }
+  }
+  __syncthreads();
+  const uint32_t bu = tile_u[blockIdx.x], bg = tile_g[blockIdx.x];
+  const uint32_t m0 = s_mesh;
+  const uint32_t nw = (uint32_t)((n_words - tw0) < BM_TILE ? (n_words - tw0) : BM_TILE);
+  // unique keys and their groups
+  for (uint32_t o = threadIdx.x; o < tu; o += BLOCK) {
+    const uint32_t i = last_le(s_upre, nw, o);
+    const uint32_t word = s_word[i];
+    const uint32_t b = select_bit(word, o - s_upre[i]);
+    const uint64_t gw = tw0 + i;
+    uint32_t m = m0;
+    while (m + 1 < n_meshes && gw >= bm_off[m + 1]) ++m;
+    const K key = ((K)m << kl.cb) | ((K)(gw - bm_off[m]) << 5) | (K)b;
+    elem1[(uint64_t)bu + o] = key;
+    group1[(uint64_t)bu + o] = bg + s_gpre[i] + nonzero_bytes_upto(word, b >> 3) - 1;
+  }
+  // groups: parent key and first child (the lowest set bit of the byte)
+  for (uint32_t q = threadIdx.x; q < tg; q += BLOCK) {
+    const uint32_t i = last_le(s_gpre, nw, q);
+    const uint32_t word = s_word[i];
+    uint32_t k = q - s_gpre[i], y = 0;
+    for (; y < 4; ++y) {
+      if ((word >> (8 * y)) & 0xFFu) {
+        if (k == 0) break;
+        --k;
+      }
+    }
+    const uint32_t b = 8 * y + (uint32_t)__builtin_ctz((word >> (8 * y)) & 0xFFu);
+    const uint64_t gw = tw0 + i;
+    uint32_t m = m0;
+    while (m + 1 < n_meshes && gw >= bm_off[m + 1]) ++m;
+    const K key = ((K)m << kl.cb) | ((K)(gw - bm_off[m]) << 5) | (K)b;
+    elem2[(uint64_t)bg + q] = kl.parent(key);
+    first_child2[(uint64_t)bg + q] = bu + s_upre[i] + __popc(word & ((1u << b) - 1u));
+  }
+}
+
+// Level-1 spans of each mesh: its ranges in the sorted unique keys and in the
+// groups (both sorted by mesh).  One wave per (mesh, bound): each step probes
+// 64 evenly spaced positions and keeps the interval where the predicate
+// "mesh of the key < target" turns false.
+template <typename K>
+__device__ inline uint32_t wave_first_of_mesh(const K* a, uint64_t n, KeyLayout<K> kl,
+                                              uint32_t target) {
+  const uint32_t lane = __lane_id();
+  uint64_t lo = 0, hi = n;  // answer in [lo, hi]
+  while (hi - lo > 64) {
+    const uint64_t step = (hi - lo + 63) / 64;
+    const uint64_t p = lo + lane * step;
+    const bool valid = p < hi;
+    const bool lt = valid && kl.mesh(a[p]) < target;
+    const uint32_t c = (uint32_t)__popcll(__ballot(lt));  // probes 0 .. c-1 are below
+    const uint64_t nlo = c == 0 ? lo : lo + (uint64_t)(c - 1) * step + 1;
+    const uint64_t cand = lo + (uint64_t)c * step;
+    const uint64_t nhi = cand < hi ? cand : hi;
+    lo = nlo;
+    hi = nhi;
+  }
+  const uint64_t p = lo + lane;
+  const bool lt = p < hi && kl.mesh(a[p]) < target;
+  return (uint32_t)(lo + (uint64_t)__popcll(__ballot(lt)));
+}
+
+template <typename K>
+__global__ __launch_bounds__(BLOCK) void k_bm_spans(const K* elem1, const K* elem2,
+                                                    const unsigned long long* counts,
+                                                    KeyLayout<K> kl, uint32_t n_meshes,
+                                                    LevelSpan* spans, uint32_t n_levels) {
+  const uint32_t m = blockIdx.x;
+  const uint32_t w = threadIdx.x >> 6;  // 0: elem_first 1: elem_end 2: group_first 3: group_end
+  const bool groups = w >= 2;
+  const uint32_t r = wave_first_of_mesh(groups ? elem2 : elem1, groups ? counts[1] : counts[0],
+                                        kl, m + (w & 1));
+  if (__lane_id() == 0) {
+    LevelSpan& sp = spans[(uint64_t)m * n_levels];
+    if (w == 0) sp.elem_first = r;
+    else if (w == 1) sp.elem_end = r;
+    else if (w == 2) sp.group_first = r;
+    else sp.group_end = r;
+  }
+}
+
 __global__ void k_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint32_t n_levels,
                                  MeshTreeState* state) {
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -380,6 +646,43 @@ void launch_level_first(const K* sorted, const unsigned long long* n_sorted, uin
                      spans1, n_levels, status, tickets, 1u, counts);
 }
 
+size_t bitmap_level_tmp_bytes(uint64_t n_words) {
+  const uint64_t tiles = (std::max<uint64_t>(n_words, 1) + BM_TILE - 1) / BM_TILE;
+  const uint64_t chunks = (tiles + SC_CHUNK - 1) / SC_CHUNK;
+  return 2 * tiles * sizeof(uint32_t) + 256 + (2 + 2 * chunks) * sizeof(uint64_t);
+}
+
+template <typename K>
+void launch_level_first_bitmap(const uint32_t* bits, uint64_t n_words, const uint64_t* bm_off,
+                               uint32_t n_meshes, KeyLayout<K> kl, LevelArrays l1,
+                               LevelArrays l2, unsigned long long* counts, LevelSpan* spans1,
+                               uint32_t n_levels, void* tmp, void* lvl_tmp, hipStream_t s) {
+  const uint64_t tiles = (std::max<uint64_t>(n_words, 1) + BM_TILE - 1) / BM_TILE;
+  const uint64_t chunks = (tiles + SC_CHUNK - 1) / SC_CHUNK;
+  uint32_t* tile_u = (uint32_t*)tmp;
+  uint32_t* tile_g = tile_u + tiles;
+  uint64_t* scan_status =
+      (uint64_t*)((char*)tmp + ((2 * tiles * sizeof(uint32_t) + 255) / 256) * 256);
+  const uint32_t n_status = (uint32_t)(2 + 2 * chunks);
+  hipLaunchKernelGGL(k_bm_count, dim3((uint32_t)tiles), dim3(BLOCK), 0, s, bits, n_words, tile_u,
+                     tile_g, scan_status, n_status);
+  hipLaunchKernelGGL(k_bm_scan, dim3((uint32_t)chunks), dim3(BLOCK), 0, s, tile_u, tile_g,
+                     (uint32_t)tiles, scan_status, counts);
+  hipLaunchKernelGGL(k_bm_emit<K>, dim3((uint32_t)tiles), dim3(BLOCK), 0, s, bits, n_words,
+                     bm_off, n_meshes, kl, (const uint32_t*)tile_u, (const uint32_t*)tile_g,
+                     (K*)l1.elem, l1.elem_group, (K*)l2.elem, l2.first_child);
+  hipLaunchKernelGGL(k_bm_spans<K>, dim3(n_meshes), dim3(BLOCK), 0, s,
+                     (const K*)l1.elem, (const K*)l2.elem, (const unsigned long long*)counts, kl,
+                     n_meshes, spans1, n_levels);
+  // the later levels' look-back words and tickets (launch_level)
+  uint32_t* tickets = (uint32_t*)lvl_tmp;
+  uint64_t* status = (uint64_t*)((char*)lvl_tmp + 256);
+  const uint64_t ltiles = (std::max<uint64_t>(l1.cap, 1) + TILE - 1) / TILE;
+  const uint32_t cg = (uint32_t)std::min<uint64_t>(2048, (2 * ltiles + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_level_clear, dim3(std::max<uint32_t>(cg, 1)), dim3(BLOCK), 0, s, status,
+                     tickets, (const unsigned long long*)counts);
+}
+
 void launch_level_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint32_t n_levels,
                                  MeshTreeState* state, hipStream_t s) {
   hipLaunchKernelGGL(k_first_finalize, dim3((n_meshes + 63) / 64), dim3(64), 0, s, spans,
@@ -407,6 +710,14 @@ template void launch_level_first<uint64_t>(const uint64_t*, const unsigned long 
                                            KeyLayout<uint64_t>, LevelArrays, LevelArrays,
                                            unsigned long long*, LevelSpan*, uint32_t, uint32_t,
                                            void*, hipStream_t);
+template void launch_level_first_bitmap<uint32_t>(const uint32_t*, uint64_t, const uint64_t*,
+                                                  uint32_t, KeyLayout<uint32_t>, LevelArrays,
+                                                  LevelArrays, unsigned long long*, LevelSpan*,
+                                                  uint32_t, void*, void*, hipStream_t);
+template void launch_level_first_bitmap<uint64_t>(const uint32_t*, uint64_t, const uint64_t*,
+                                                  uint32_t, KeyLayout<uint64_t>, LevelArrays,
+                                                  LevelArrays, unsigned long long*, LevelSpan*,
+                                                  uint32_t, void*, void*, hipStream_t);
 template void launch_level<uint32_t>(int, KeyLayout<uint32_t>, LevelArrays, LevelArrays,
                                      unsigned long long*, LevelSpan*, uint32_t, MeshTreeState*,
                                      uint32_t, void*, hipStream_t);

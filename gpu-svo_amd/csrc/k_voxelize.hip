@@ -333,6 +333,53 @@ struct Stager {
   }
 };
 
+// Occupancy-bitmap output: bit (code & 31) of word bm_off[mesh] + code / 32
+// is set for every hit (the dense flag array of SVO.cu:180-182 at one bit per
+// cell).  Each lane keeps the word of its last hit and ORs it into memory
+// only when a hit falls in another word (consecutive cells of a row share a
+// word up to four at a time), so the atomics issued are fewer than the hits.
+// Raw hits are counted per wave and added once at the end.
+struct BitSink {
+  uint32_t* bits;
+  unsigned long long* count;
+  uint64_t n = 0;          // wave-uniform
+  uint64_t pend_word = 0;  // per lane
+  uint32_t pend_bits = 0;
+  __device__ inline void flush_lane() {
+    if (pend_bits)
+      __hip_atomic_fetch_or(bits + pend_word, pend_bits, __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __device__ inline void push(bool hit, uint64_t word, uint32_t bit) {
+    const uint64_t mask = __ballot(hit);
+    if (mask == 0) return;
+    n += (uint64_t)__popcll(mask);
+    if (hit) {
+      if (word != pend_word) {
+        flush_lane();
+        pend_word = word;
+        pend_bits = 0;
+      }
+      pend_bits |= 1u << bit;
+    }
+  }
+  __device__ inline void finish() {
+    flush_lane();
+    pend_bits = 0;
+    if (n && __lane_id() == 0) atomicAdd(count, (unsigned long long)n);
+  }
+};
+
+// Where a voxelization kernel puts its hits: keys (staged, for the sort) or
+// the occupancy bitmap.
+template <typename K>
+struct VoxOut {
+  K* keys;                // key path
+  uint32_t* bits;         // bitmap path
+  const uint64_t* bm_off; // bitmap words before each mesh
+  unsigned long long* count;
+};
+
 // Big-triangle work item: 256 rows (y, z) of the box, four per lane.
 constexpr uint64_t BIG_ROWS = 256;
 
@@ -385,20 +432,24 @@ __global__ void k_flag_bad_index(const MeshDev* meshes, MeshParams* params,
     if (t[k] < 0 || (uint64_t)t[k] >= md.n_points) atomicOr(&params[m].status, ERR_INDEX);
 }
 
-template <typename K>
+template <typename K, bool BITMAP>
 __global__ __launch_bounds__(BLOCK) void k_vox_small(
     const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base, uint32_t n_meshes,
-    uint64_t n_tris, int cb, uint32_t small_max, uint32_t chunk_cells, K* keys,
-    unsigned long long* key_count, BigItem* big, unsigned long long* big_count, uint64_t big_cap) {
-  __shared__ K stage[WAVES][STAGE];
+    uint64_t n_tris, int cb, uint32_t small_max, uint32_t chunk_cells, VoxOut<K> out,
+    BigItem* big, unsigned long long* big_count, uint64_t big_cap) {
+  __shared__ K stage[BITMAP ? 1 : WAVES][BITMAP ? 1 : STAGE];
   Stager<K> st;
-  st.buf = stage[threadIdx.x >> 6];
-  st.keys = keys;
-  st.count = key_count;
+  st.buf = stage[BITMAP ? 0 : threadIdx.x >> 6];
+  st.keys = out.keys;
+  st.count = out.count;
+  BitSink bs;
+  bs.bits = out.bits;
+  bs.count = out.count;
   const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
   TriSetup S;
   uint64_t count = 0;
   K mesh_key = 0;
+  uint64_t wbase = 0;
   if (gid < n_tris) {
     const uint32_t m = find_mesh(tri_base, n_meshes, gid);
     const MeshDev& md = meshes[m];
@@ -418,6 +469,7 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
           if (at + j < big_cap) big[at + j] = BigItem{(uint32_t)gid, (uint32_t)(j * BIG_ROWS)};
       }
       mesh_key = (K)m << cb;
+      if (BITMAP) wbase = out.bm_off[m];
     }
   }
   uint64_t c = 0;
@@ -426,12 +478,12 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
   RowTerms R;
   while (__any(c < count)) {
     bool hit = false;
-    K key = 0;
+    K code = 0;
     if (c < count) {
       if (x == S.g0[0]) row_ok = row_terms(S, y, z, R);
       if (row_ok) {
         hit = cell_hit(S, R, x);
-        key = mesh_key | morton3<K>((uint32_t)x, (uint32_t)y, (uint32_t)z);
+        code = morton3<K>((uint32_t)x, (uint32_t)y, (uint32_t)z);
       }
       if (row_ok && x < S.g1[0]) {
         ++x;
@@ -445,31 +497,33 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
         }
       }
     }
-    st.push(hit, key);
+    if (BITMAP) bs.push(hit, wbase + (uint64_t)(code >> 5), (uint32_t)(code & 31));
+    else st.push(hit, mesh_key | code);
   }
-  st.finish();
+  if (BITMAP) bs.finish();
+  else st.finish();
 }
 
 // One wave per work item of 256 rows: each lane prunes four rows of the item
 // to their x-ranges and walks them; each inner loop runs as long as the
 // longest pruned row of the 64.
-template <typename K>
+template <typename K, bool BITMAP>
 __global__ __launch_bounds__(BLOCK) void k_vox_big(const MeshDev* meshes,
                                                    const MeshParams* params,
                                                    const uint64_t* tri_base, uint32_t n_meshes,
-                                                   int cb, uint32_t chunk_cells,
-                                                   const BigItem* big, uint64_t big_cap,
+                                                   int cb, const BigItem* big, uint64_t big_cap,
                                                    const unsigned long long* big_count,
-                                                   uint64_t n_tris, K* keys,
-                                                   unsigned long long* key_count) {
-  (void)chunk_cells;
-  __shared__ K stage[WAVES][STAGE];
+                                                   uint64_t n_tris, VoxOut<K> out) {
+  __shared__ K stage[BITMAP ? 1 : WAVES][BITMAP ? 1 : STAGE];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = __lane_id();
   Stager<K> st;
-  st.buf = stage[w];
-  st.keys = keys;
-  st.count = key_count;
+  st.buf = stage[BITMAP ? 0 : w];
+  st.keys = out.keys;
+  st.count = out.count;
+  BitSink bs;
+  bs.bits = out.bits;
+  bs.count = out.count;
   const uint64_t item_id = (uint64_t)blockIdx.x * WAVES + w;
   const uint64_t queued = *big_count;
   const uint64_t n_items = queued < big_cap ? queued : big_cap;
@@ -484,6 +538,7 @@ __global__ __launch_bounds__(BLOCK) void k_vox_big(const MeshDev* meshes,
     tri_setup(v, P, S);
     const uint64_t ny = S.g1[1] - S.g0[1] + 1, nz = S.g1[2] - S.g0[2] + 1;
     const K mesh_key = (K)m << cb;
+    const uint64_t wbase = BITMAP ? out.bm_off[m] : 0;
     for (uint64_t k = 0; k < BIG_ROWS; k += 64) {
       const uint64_t r = (uint64_t)it.row0 + k + lane;
       int y = 0, z = 0, xa = 0, xb = -1;
@@ -495,11 +550,14 @@ __global__ __launch_bounds__(BLOCK) void k_vox_big(const MeshDev* meshes,
       }
       for (int x = xa; __any(x <= xb); ++x) {
         const bool hit = x <= xb && cell_hit(S, R, x);
-        st.push(hit, mesh_key | morton3<K>((uint32_t)x, (uint32_t)y, (uint32_t)z));
+        const K code = morton3<K>((uint32_t)x, (uint32_t)y, (uint32_t)z);
+        if (BITMAP) bs.push(hit, wbase + (uint64_t)(code >> 5), (uint32_t)(code & 31));
+        else st.push(hit, mesh_key | code);
       }
     }
   }
-  st.finish();
+  if (BITMAP) bs.finish();
+  else st.finish();
 }
 
 }  // namespace
@@ -533,27 +591,39 @@ void launch_classify(const MeshDev* meshes, const MeshParams* params, const uint
 template <typename K>
 void launch_voxelize(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
                      uint32_t n_meshes, uint64_t n_tris, int cb, const Tuning& tn, K* keys,
-                     unsigned long long* key_count, BigItem* big, unsigned long long* big_count,
-                     uint64_t big_cap, hipStream_t s) {
+                     uint32_t* bitmap, const uint64_t* bm_off, unsigned long long* key_count,
+                     BigItem* big, unsigned long long* big_count, uint64_t big_cap,
+                     hipStream_t s) {
   if (n_tris == 0) return;
+  const VoxOut<K> out{keys, bitmap, bm_off, key_count};
   const uint32_t blocks = (uint32_t)((n_tris + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL(k_vox_small<K>, dim3(blocks), dim3(BLOCK), 0, s, meshes, params, tri_base,
-                     n_meshes, n_tris, cb, tn.small_max_cells, tn.chunk_cells, keys, key_count,
-                     big, big_count, big_cap);
-  if (big_cap == 0) return;
   const uint32_t bblocks = (uint32_t)((big_cap + WAVES - 1) / WAVES);
-  hipLaunchKernelGGL(k_vox_big<K>, dim3(bblocks), dim3(BLOCK), 0, s, meshes, params, tri_base,
-                     n_meshes, cb, tn.chunk_cells, (const BigItem*)big, big_cap,
-                     (const unsigned long long*)big_count, n_tris, keys, key_count);
+  if (bitmap) {
+    hipLaunchKernelGGL((k_vox_small<K, true>), dim3(blocks), dim3(BLOCK), 0, s, meshes, params,
+                       tri_base, n_meshes, n_tris, cb, tn.small_max_cells, tn.chunk_cells, out,
+                       big, big_count, big_cap);
+    if (big_cap)
+      hipLaunchKernelGGL((k_vox_big<K, true>), dim3(bblocks), dim3(BLOCK), 0, s, meshes, params,
+                         tri_base, n_meshes, cb, (const BigItem*)big, big_cap,
+                         (const unsigned long long*)big_count, n_tris, out);
+  } else {
+    hipLaunchKernelGGL((k_vox_small<K, false>), dim3(blocks), dim3(BLOCK), 0, s, meshes, params,
+                       tri_base, n_meshes, n_tris, cb, tn.small_max_cells, tn.chunk_cells, out,
+                       big, big_count, big_cap);
+    if (big_cap)
+      hipLaunchKernelGGL((k_vox_big<K, false>), dim3(bblocks), dim3(BLOCK), 0, s, meshes, params,
+                         tri_base, n_meshes, cb, (const BigItem*)big, big_cap,
+                         (const unsigned long long*)big_count, n_tris, out);
+  }
 }
 
 template void launch_voxelize<uint32_t>(const MeshDev*, const MeshParams*, const uint64_t*,
                                         uint32_t, uint64_t, int, const Tuning&, uint32_t*,
-                                        unsigned long long*, BigItem*, unsigned long long*,
-                                        uint64_t, hipStream_t);
+                                        uint32_t*, const uint64_t*, unsigned long long*,
+                                        BigItem*, unsigned long long*, uint64_t, hipStream_t);
 template void launch_voxelize<uint64_t>(const MeshDev*, const MeshParams*, const uint64_t*,
                                         uint32_t, uint64_t, int, const Tuning&, uint64_t*,
-                                        unsigned long long*, BigItem*, unsigned long long*,
-                                        uint64_t, hipStream_t);
+                                        uint32_t*, const uint64_t*, unsigned long long*,
+                                        BigItem*, unsigned long long*, uint64_t, hipStream_t);
 
 }  // namespace svo

@@ -130,11 +130,11 @@ int svo_k_voxelize(int device, const float* xyz, uint64_t n_points, const int32_
       st = SVO_E_OOM;
     } else {
       if (kb == 64)
-        launch_voxelize<uint64_t>(md, par, tb, 1, n_tris, 3 * bits, tn, (uint64_t*)keys, cnt + 2,
-                                  big, cnt + 3, n_big, s);
+        launch_voxelize<uint64_t>(md, par, tb, 1, n_tris, 3 * bits, tn, (uint64_t*)keys, nullptr,
+                                  nullptr, cnt + 2, big, cnt + 3, n_big, s);
       else
-        launch_voxelize<uint32_t>(md, par, tb, 1, n_tris, 3 * bits, tn, (uint32_t*)keys, cnt + 2,
-                                  big, cnt + 3, n_big, s);
+        launch_voxelize<uint32_t>(md, par, tb, 1, n_tris, 3 * bits, tn, (uint32_t*)keys, nullptr,
+                                  nullptr, cnt + 2, big, cnt + 3, n_big, s);
       st = SVO_E_HIP;
       if (hipMemcpyAsync(tot, cnt, 32, hipMemcpyDeviceToHost, s) == hipSuccess &&
           hipStreamSynchronize(s) == hipSuccess) {

@@ -180,8 +180,14 @@ DeviceBlock::~DeviceBlock() {
 int Workspace::get(const char* name, size_t bytes, void** out) {
   bytes = align256(bytes ? bytes : 256);
   Buf& b = bufs[name];
-  if (b.bytes < bytes) {
-    if (b.ptr) ctx->dev_free(b.ptr, b.bytes);
+  // grow when too small; give back a buffer far larger than this pass needs
+  // (after one huge pass the lane would otherwise hold its memory for good)
+  const bool oversize = b.bytes > 4 * bytes && b.bytes - bytes > (1ull << 30);
+  if (b.bytes < bytes || oversize) {
+    if (b.ptr) {
+      ctx->dev_free(b.ptr, b.bytes);
+      ctx->ws_bytes.fetch_sub(b.bytes);
+    }
     b.ptr = nullptr;
     b.bytes = 0;
     // grow with headroom so that slowly growing inputs do not reallocate
@@ -196,6 +202,7 @@ int Workspace::get(const char* name, size_t bytes, void** out) {
       b.bytes = want;
     }
     b.ptr = p;
+    ctx->ws_bytes.fetch_add(b.bytes);
   }
   *out = b.ptr;
   return SVO_OK;
@@ -203,7 +210,10 @@ int Workspace::get(const char* name, size_t bytes, void** out) {
 
 void Workspace::release_all() {
   for (auto& kv : bufs)
-    if (kv.second.ptr) ctx->dev_free(kv.second.ptr, kv.second.bytes);
+    if (kv.second.ptr) {
+      ctx->dev_free(kv.second.ptr, kv.second.bytes);
+      ctx->ws_bytes.fetch_sub(kv.second.bytes);
+    }
   bufs.clear();
 }
 
@@ -393,26 +403,43 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   }
   const uint64_t T = tri_base[n];
 
-  // small control buffers: mesh descriptors, tri bases, bbox, params, counters
+  // occupancy-bitmap words per mesh: one bit per Morton code below
+  // morton(res - 1) + 1 (the reference's dense array size, SVO.cu:342-345)
+  std::vector<uint64_t> bm_off(n + 1, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    const MeshDev& d = md[i];
+    uint64_t g = 1;
+    if (d.res[0] > 0 && d.res[1] > 0 && d.res[2] > 0)
+      g = (uint64_t)morton3<uint64_t>((uint32_t)d.res[0] - 1, (uint32_t)d.res[1] - 1,
+                                      (uint32_t)d.res[2] - 1) + 1;
+    bm_off[i + 1] = bm_off[i] + (g + 31) / 32;
+  }
+  const uint64_t bm_words = bm_off[n];
+
+  // small control buffers: mesh descriptors, tri bases, bitmap offsets, bbox,
+  // params, counters
   const size_t sz_md = align256(sizeof(MeshDev) * n), sz_tb = align256(8 * (n + 1)),
-               sz_acc = align256(32 * n), sz_par = align256(sizeof(MeshParams) * n),
-               sz_cnt = align256(8 * 8);
+               sz_bo = align256(8 * (n + 1)), sz_acc = align256(32 * n),
+               sz_par = align256(sizeof(MeshParams) * n), sz_cnt = align256(8 * 8);
   char* ctl = nullptr;
-  SVO_TRY(W.get("ctl", sz_md + sz_tb + sz_acc + sz_par + sz_cnt, (void**)&ctl));
+  SVO_TRY(W.get("ctl", sz_md + sz_tb + sz_bo + sz_acc + sz_par + sz_cnt, (void**)&ctl));
   MeshDev* d_md = (MeshDev*)ctl;
   uint64_t* d_tb = (uint64_t*)(ctl + sz_md);
-  uint32_t* d_acc = (uint32_t*)(ctl + sz_md + sz_tb);
-  MeshParams* d_par = (MeshParams*)(ctl + sz_md + sz_tb + sz_acc);
-  unsigned long long* d_cnt = (unsigned long long*)(ctl + sz_md + sz_tb + sz_acc + sz_par);
+  uint64_t* d_bo = (uint64_t*)(ctl + sz_md + sz_tb);
+  uint32_t* d_acc = (uint32_t*)(ctl + sz_md + sz_tb + sz_bo);
+  MeshParams* d_par = (MeshParams*)(ctl + sz_md + sz_tb + sz_bo + sz_acc);
+  unsigned long long* d_cnt =
+      (unsigned long long*)(ctl + sz_md + sz_tb + sz_bo + sz_acc + sz_par);
   // d_cnt: [0] box cells, [1] big chunks, [2] keys emitted, [3] big queued
-  SVO_TRY(L.ensure_pinned(sz_md + sz_tb + 64 + sizeof(MeshParams) * n));
+  SVO_TRY(L.ensure_pinned(sz_md + sz_tb + sz_bo + 64 + sizeof(MeshParams) * n));
   {
-    // stage the mesh descriptors and triangle bases through the pinned
-    // buffer in one copy; the host does not touch the buffer again before
-    // the next stream synchronization
+    // stage the mesh descriptors, triangle bases and bitmap offsets through
+    // the pinned buffer in one copy; the host does not touch the buffer again
+    // before the next stream synchronization
     std::memcpy(L.pinned, md.data(), sizeof(MeshDev) * n);
     std::memcpy((char*)L.pinned + sz_md, tri_base.data(), 8 * (n + 1));
-    HIP_TRY(hipMemcpyAsync(d_md, L.pinned, sz_md + sz_tb, hipMemcpyHostToDevice, s));
+    std::memcpy((char*)L.pinned + sz_md + sz_tb, bm_off.data(), 8 * (n + 1));
+    HIP_TRY(hipMemcpyAsync(d_md, L.pinned, sz_md + sz_tb + sz_bo, hipMemcpyHostToDevice, s));
   }
   launch_fill_u32((uint32_t*)d_cnt, 0u, 16, s);
   timer.mark("start");
@@ -429,24 +456,41 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   if (cells >= (1ull << 32) - 1) return SVO_E_OOM;  // key counts are 32-bit in the sort
   const uint64_t key_cap = std::max<uint64_t>(cells, 1);
 
+  // ---- de-duplication path: occupancy bitmap (level 1 read straight from
+  // it) while it is small enough, else a radix sort of the raw hits
+  const uint64_t bm_bytes = 4 * bm_words;
+  bool use_bm = bm_bytes <= C.tuning.bitmap_max_bytes &&
+                bm_bytes <= (uint64_t)C.tuning.bitmap_bytes_per_cell * key_cap;
+  if (C.tuning.force_path == 1) use_bm = true;
+  if (C.tuning.force_path == 2) use_bm = false;
+  // level-1 capacity: unique voxels <= min(candidate cells, bitmap bits)
+  const uint64_t u_cap = use_bm ? std::max<uint64_t>(1, std::min<uint64_t>(key_cap, 32 * bm_words))
+                                : key_cap;
+
   // ---- workspace for A1
   K *keys_a = nullptr, *keys_b = nullptr;
+  uint32_t* bitmap = nullptr;
   BigItem* big = nullptr;
-  void *sort_tmp = nullptr, *lvl_tmp = nullptr;
-  SVO_TRY(W.get("keys_a", sizeof(K) * key_cap, (void**)&keys_a));
-  SVO_TRY(W.get("keys_b", sizeof(K) * key_cap, (void**)&keys_b));
+  void *sort_tmp = nullptr, *lvl_tmp = nullptr, *bm_tmp = nullptr;
+  if (use_bm) {
+    SVO_TRY(W.get("bitmap", bm_bytes, (void**)&bitmap));
+    SVO_TRY(W.get("bm_tmp", bitmap_level_tmp_bytes(bm_words), &bm_tmp));
+  } else {
+    SVO_TRY(W.get("keys_a", sizeof(K) * key_cap, (void**)&keys_a));
+    SVO_TRY(W.get("keys_b", sizeof(K) * key_cap, (void**)&keys_b));
+  }
   SVO_TRY(W.get("big", sizeof(BigItem) * std::max<uint64_t>(n_big, 1), (void**)&big));
   // fixed grids sized from host-known capacities (small inputs get small grids)
   auto grid_for = [](uint64_t cap_elems, uint64_t per_block, uint32_t max_blocks) {
     const uint64_t b = (cap_elems + per_block - 1) / per_block;
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, max_blocks));
   };
-  SVO_TRY(W.get("sort_tmp", radix_sort_tmp_bytes(key_cap), &sort_tmp));
-  SVO_TRY(W.get("lvl_tmp", level_tmp_bytes(key_cap), &lvl_tmp));
-  // level capacities: cap_1 = box cells (>= hits >= unique voxels);
-  // cap_l = min(cap_{l-1}, number of level-l cells over the batch)
+  if (!use_bm) SVO_TRY(W.get("sort_tmp", radix_sort_tmp_bytes(key_cap), &sort_tmp));
+  SVO_TRY(W.get("lvl_tmp", level_tmp_bytes(use_bm ? u_cap : key_cap), &lvl_tmp));
+  // level capacities: cap_1 = unique voxels bound (box cells >= hits >=
+  // unique voxels); cap_l = min(cap_{l-1}, number of level-l cells)
   std::vector<uint64_t> cap(NL + 2, 0);
-  cap[1] = key_cap;
+  cap[1] = u_cap;
   for (uint32_t l = 2; l <= NL + 1; ++l) {
     uint64_t cellsl = 0;
     for (uint32_t i = 0; i < n; ++i) {
@@ -492,15 +536,24 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
 
   // ---- A1: voxelize, sort, levels
   timer.mark("setup_a1");
-  launch_voxelize<K>(d_md, d_par, d_tb, n, T, plan.cb, C.tuning, keys_a, d_cnt + 2, big,
-                     d_cnt + 3, n_big, s);
+  if (use_bm) {
+    launch_fill_u32(bitmap, 0u, bm_words, s);
+    timer.mark("bitmap_clear");
+  }
+  launch_voxelize<K>(d_md, d_par, d_tb, n, T, plan.cb, C.tuning, keys_a, bitmap, d_bo,
+                     d_cnt + 2, big, d_cnt + 3, n_big, s);
   timer.mark("voxelize");
-  K* sorted = nullptr;
-  launch_radix_sort<K>(keys_a, keys_b, d_cnt + 2, key_cap, plan.cb + plan.mb, C.tuning.sort_blocks,
-                       sort_tmp, &sorted, s);
-  timer.mark("sort");
-  launch_level_first<K>(sorted, d_cnt + 2, key_cap, kl, lv[1], lv[2], d_counts, d_spans, NL,
-                        C.tuning.scan_blocks, lvl_tmp, s);
+  if (use_bm) {
+    launch_level_first_bitmap<K>(bitmap, bm_words, d_bo, n, kl, lv[1], lv[2], d_counts, d_spans,
+                                 NL, bm_tmp, lvl_tmp, s);
+  } else {
+    K* sorted = nullptr;
+    launch_radix_sort<K>(keys_a, keys_b, d_cnt + 2, key_cap, plan.cb + plan.mb,
+                         C.tuning.sort_blocks, sort_tmp, &sorted, s);
+    timer.mark("sort");
+    launch_level_first<K>(sorted, d_cnt + 2, key_cap, kl, lv[1], lv[2], d_counts, d_spans, NL,
+                          C.tuning.scan_blocks, lvl_tmp, s);
+  }
   launch_level_first_finalize(d_spans, n, NL, d_state, s);
   timer.mark("unique_level1");
   for (uint32_t l = 2; l <= NL; ++l)
@@ -684,7 +737,8 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     ps.out_bytes = out->block->bytes;
     ps.key_bytes = sizeof(K);
     ps.sort_bits = (uint32_t)(plan.cb + plan.mb);
-    ps.sort_passes = (ps.sort_bits + 7) / 8;
+    ps.sort_passes = use_bm ? 0 : (ps.sort_bits + 7) / 8;
+    ps.bitmap_bytes = use_bm ? bm_bytes : 0;
     ps.n_levels = NL;
     std::lock_guard<std::mutex> lk(C.stage_mu);
     C.last_pass = ps;

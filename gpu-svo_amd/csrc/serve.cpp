@@ -71,7 +71,10 @@ void svo_server::worker_loop() {
     // admission against the budget
     std::vector<Request> batch;
     uint64_t est = 0;
-    const uint64_t resident = ctx->in_use.load();
+    // resident trees; lane workspaces are reused by the batch and are part
+    // of its estimate
+    const uint64_t used = ctx->in_use.load(), ws = ctx->ws_bytes.load();
+    const uint64_t resident = used > ws ? used - ws : 0;
     while (!pending.empty() && batch.size() < cfg.max_batch) {
       const uint64_t e = pending.front().estimate;
       if (cfg.mem_budget && resident + in_flight_bytes + est + e > cfg.mem_budget &&

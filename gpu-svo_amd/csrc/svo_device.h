@@ -81,6 +81,14 @@ __host__ __device__ inline void demorton3(K m, uint32_t& x, uint32_t& y, uint32_
   }
 }
 
+// Streaming 16-byte store for output written once and not read back by the
+// same kernel (non-temporal hint: no need to keep the lines in the caches).
+typedef uint32_t u32x4_nt __attribute__((ext_vector_type(4)));
+__device__ inline void store_stream(uint4* p, uint4 v) {
+  u32x4_nt x = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(x, reinterpret_cast<u32x4_nt*>(p));
+}
+
 // ---------------------------------------------------------------- keys
 template <typename K>
 struct KeyLayout {

@@ -16,7 +16,9 @@ namespace svo {
 
 struct Context;
 
-// Grow-only device buffers of one pipeline lane.
+// Named device buffers of one pipeline lane, kept between passes: a buffer
+// grows when a pass needs more and is given back when it is far larger than
+// a pass needs.
 struct Workspace {
   struct Buf {
     void* ptr = nullptr;
@@ -61,6 +63,7 @@ struct Context {
   int device = 0;
   uint64_t budget = 0;  // 0 = unbounded
   std::atomic<uint64_t> in_use{0};
+  std::atomic<uint64_t> ws_bytes{0};  // part of in_use held by lane workspaces (reused)
   Tuning tuning;
   std::mutex mu;
   std::condition_variable cv;

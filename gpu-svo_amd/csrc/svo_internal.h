@@ -15,6 +15,12 @@ struct Tuning {
   uint32_t chunk_cells = 4096;      // cells per big-triangle work item
   uint32_t sort_blocks = 1024;      // persistent grid bound of the radix sort passes
   uint32_t scan_blocks = 1024;      // fixed grid of the level / attribute scans
+  // Occupancy bitmap instead of the sort (level 1 straight from the bitmap)
+  // while the bitmap is at most this many bytes and at most this many bytes
+  // per candidate cell (so sparse hits in a huge grid still take the sort).
+  uint64_t bitmap_max_bytes = 32ull << 30;
+  uint32_t bitmap_bytes_per_cell = 64;
+  int force_path = 0;               // 0 automatic, 1 bitmap, 2 sort
 };
 
 // ------------------------------------------------------------ k_voxelize.hip
@@ -30,10 +36,14 @@ void launch_classify(const MeshDev* meshes, const MeshParams* params, const uint
                      unsigned long long* totals, hipStream_t s);
 // Surface voxelization (SVO.cu:37-188): emits composite keys.
 template <typename K>
+// Hits go to `keys` (raw, for the sort) when `bitmap` is null, else into the
+// occupancy bitmap (word bm_off[mesh] + code / 32, bit code % 32, which the
+// caller has zeroed); *key_count receives the number of raw hits.
 void launch_voxelize(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
                      uint32_t n_meshes, uint64_t n_tris, int cb, const Tuning& tn, K* keys,
-                     unsigned long long* key_count, BigItem* big, unsigned long long* big_count,
-                     uint64_t big_cap, hipStream_t s);
+                     uint32_t* bitmap, const uint64_t* bm_off, unsigned long long* key_count,
+                     BigItem* big, unsigned long long* big_count, uint64_t big_cap,
+                     hipStream_t s);
 
 // ------------------------------------------------------------ k_sort.hip
 // LSD radix sort of n (device-resident count) keys on bits [0, bits).
@@ -62,6 +72,16 @@ void launch_level_first(const K* sorted, const unsigned long long* n_sorted, uin
                         KeyLayout<K> kl, LevelArrays l1, LevelArrays l2,
                         unsigned long long* counts, LevelSpan* spans1, uint32_t n_levels,
                         uint32_t blocks, void* tmp, hipStream_t s);
+// Level 1 from the occupancy bitmap (n_words words; mesh m owns words
+// [bm_off[m], bm_off[m+1])): the same outputs as launch_level_first.  `tmp`
+// needs bitmap_level_tmp_bytes(n_words); `lvl_tmp` (level_tmp_bytes of
+// l1.cap) is prepared for the later levels.
+size_t bitmap_level_tmp_bytes(uint64_t n_words);
+template <typename K>
+void launch_level_first_bitmap(const uint32_t* bits, uint64_t n_words, const uint64_t* bm_off,
+                               uint32_t n_meshes, KeyLayout<K> kl, LevelArrays l1,
+                               LevelArrays l2, unsigned long long* counts, LevelSpan* spans1,
+                               uint32_t n_levels, void* tmp, void* lvl_tmp, hipStream_t s);
 // Marks meshes whose level-1 set holds a single voxel as depth-1 trees.
 void launch_level_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint32_t n_levels,
                                  MeshTreeState* state, hipStream_t s);

@@ -84,6 +84,8 @@ _SIGS = {
     "svo_ctx_create": (C.c_int, [C.c_int, C.c_uint64, C.POINTER(C.c_void_p)]),
     "svo_ctx_destroy": (C.c_int, [C.c_void_p]),
     "svo_ctx_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "svo_ctx_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "svo_ctx_get_option": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64)]),
     "svo_ctx_last_stage_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_char_p), C.c_int,
                                         C.POINTER(C.c_int)]),
@@ -348,6 +350,15 @@ class Context:
 
     def set_timing(self, on: bool = True):
         lib().svo_ctx_set_timing(self._h, 1 if on else 0)
+
+    def set_option(self, name: str, value: int):
+        """Tuning option (svo_ctx_set_option); set between builds."""
+        _check(lib().svo_ctx_set_option(self._h, name.encode(), int(value)), f"option {name}")
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int64()
+        _check(lib().svo_ctx_get_option(self._h, name.encode(), C.byref(v)), f"option {name}")
+        return v.value
 
     def stage_ms(self) -> dict:
         n = C.c_int()

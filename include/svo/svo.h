@@ -151,11 +151,23 @@ int svo_ctx_destroy(svo_ctx* ctx);
  * measured with HIP events on the build's stream when timing is enabled. */
 int svo_ctx_set_timing(svo_ctx* ctx, int enable);
 int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap, int* n);
+/* Tuning options of the context (no counterpart in the reference), set
+ * between builds:
+ *   "small_max_cells"        box cells up to which one lane voxelizes a triangle (512)
+ *   "sort_blocks"            persistent grid bound of the radix sort (1024)
+ *   "scan_blocks"            persistent grid bound of the level passes (1024)
+ *   "bitmap_max_bytes"       largest occupancy bitmap used instead of the sort (32 GiB)
+ *   "bitmap_bytes_per_cell"  bitmap bytes allowed per candidate cell (64)
+ *   "dedup_path"             0 automatic, 1 always the bitmap, 2 always the sort
+ * Unknown names or out-of-range values give SVO_E_INVALID. */
+int svo_ctx_set_option(svo_ctx* ctx, const char* name, int64_t value);
+int svo_ctx_get_option(svo_ctx* ctx, const char* name, int64_t* value);
 /* Sizes of the last pipeline pass on this context (all meshes of the pass). */
 typedef struct {
   uint64_t n_meshes, n_tris, box_cells, n_hits, n_voxels, level_elems, n_nodes;
   uint64_t n_vertices, n_edges, n_faces, out_bytes;
   uint32_t key_bytes, sort_bits, sort_passes, n_levels;
+  uint64_t bitmap_bytes; /* occupancy bitmap of the pass; 0 when the hits were sorted */
 } svo_pass_stats;
 int svo_ctx_last_pass_stats(svo_ctx* ctx, svo_pass_stats* out);
 uint64_t svo_ctx_bytes_in_use(const svo_ctx* ctx);

@@ -23,11 +23,16 @@ def _meshes(n, seed, res_choices=(64, 128)):
     return out
 
 
+@pytest.mark.parametrize("path", [1, 2], ids=["bitmap", "sort"])
 @pytest.mark.parametrize("sem", [0, 1])
-def test_batch_equals_plain_and_oracle(gpu_ctx, sem):
+def test_batch_equals_plain_and_oracle(gpu_ctx, sem, path):
     meshes = _meshes(40, seed=21)
-    print(f"batch of {len(meshes)} meshes, semantics {sem}", flush=True)
-    trees, st = gpu_ctx.build_batch(meshes, semantics=sem)
+    print(f"batch of {len(meshes)} meshes, semantics {sem}, dedup path {path}", flush=True)
+    gpu_ctx.set_option("dedup_path", path)
+    try:
+        trees, st = gpu_ctx.build_batch(meshes, semantics=sem)
+    finally:
+        gpu_ctx.set_option("dedup_path", 0)
     assert all(s == 0 for s in st), st
     for i, ((P, T, res), tb) in enumerate(zip(meshes, trees)):
         tp = gpu_ctx.build(P, T, res, semantics=sem)

@@ -139,3 +139,34 @@ def test_obj_files_byte_exact(gpu_ctx, bunny, tmp_path, res, kb):
     assert open(path, "rb").read() == open(ref_t, "rb").read()
     assert open(vpath, "rb").read() == open(ref_v, "rb").read()
     tree.free()
+
+
+@pytest.mark.parametrize("path", [1, 2], ids=["bitmap", "sort"])
+def test_dedup_paths(gpu_ctx, bunny, path):
+    """Both de-duplication paths (occupancy bitmap, radix sort of the hits)
+    give the oracle's tree."""
+    gpu_ctx.set_option("dedup_path", path)
+    assert gpu_ctx.get_option("dedup_path") == path
+    try:
+        P, T = bunny
+        _case(gpu_ctx, P, T, 64)
+        _case(gpu_ctx, P, T, (40, 24, 33), sem=1)
+        Ps, Ts = _meshgen().soup(4000, seed=7, median=1.0 / 32.0)
+        _case(gpu_ctx, Ps, Ts, 256)
+        _case(gpu_ctx, Ps, Ts, 100, kb=64)
+        P1 = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0.5]], np.float32)
+        T1 = np.array([[0, 1, 2]], np.int32)
+        for res in (1, 2, 7):
+            _case(gpu_ctx, P1, T1, res)
+        Pl, Tl = _slivers(20, 3)
+        _case(gpu_ctx, Pl, Tl, 2048, kb=64)
+    finally:
+        gpu_ctx.set_option("dedup_path", 0)
+
+
+def test_options_rejected(gpu_ctx):
+    import svo_amd
+    for name, value in (("dedup_path", 3), ("no_such_option", 1), ("sort_blocks", 0)):
+        with pytest.raises(svo_amd.SvoError) as e:
+            gpu_ctx.set_option(name, value)
+        assert e.value.status == svo_amd.E_INVALID
