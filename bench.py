@@ -57,12 +57,24 @@ def stage_bytes(stats, name):
     if name == "neighbor_fill":
         return 108 * N
     if name == "nodes":
-        return 57 * N + 4 * N
+        # records written (code, is_leaf, origin, width, parent, childs), the
+        # level keys read, and the voxel codes copied out
+        return (2 * kb + 53) * N + 2 * kb * U
     if name == "attr_write":
-        return 16 * stats["n_vertices"] + 28 * stats["n_edges"] + 28 * stats["n_faces"] + 32 * N
+        # vertex / edge / face records written, node origin and width read
+        return 16 * stats["n_vertices"] + 28 * stats["n_edges"] + 28 * stats["n_faces"] + 16 * N
     if name == "attr_count":
         return 32 * N
     return 0
+
+
+# Measured HBM traffic per algorithmic byte of each stage's kernels on the
+# default workload (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch,
+# scaled by the same counters on a device-to-device copy of 1 GiB: FETCH_SIZE
+# x 2, WRITE_SIZE x 1; profiles/batch256_hbm_traffic.md).  Used to fill
+# roofline.traffic for this run's launch.
+TRAFFIC_PER_ALG_BYTE = {"attr_write": 1.000, "neighbor_fill": 1.000, "nodes": 1.00,
+                        "unique_level1": 1.00, "voxelize": 2.53}
 
 
 def dedup_gbs(stats, stages):
@@ -139,8 +151,12 @@ def run_batch_workload(args):
     n_tris = sum(int(T.shape[0]) for _, T, _ in meshes)
     ctx = svo_amd.Context(0)
     ctx.set_timing(True)
+    if args.batch_lanes:
+        ctx.set_option("batch_lanes", args.batch_lanes)
     dms = [ctx.upload_mesh(P, T) for P, T, _ in meshes]
     descs = [dm.desc(res) for dm, (_, _, res) in zip(dms, meshes)]
+
+    sizes = {}
 
     def batch_step():
         trees, st = ctx.build_batch(descs, on_device=True)
@@ -149,6 +165,8 @@ def run_batch_workload(args):
             raise RuntimeError(f"batched build failed: statuses {set(bad)}")
         nv = sum(t.info.n_voxels for t in trees)
         nn = sum(t.info.n_nodes for t in trees)
+        sizes["vertices"] = sum(t.info.n_vertices for t in trees)
+        sizes["edges"] = sum(t.info.n_edges for t in trees)
         for t in trees:
             t.free()
         return nv, nn
@@ -186,11 +204,6 @@ def run_batch_workload(args):
     t1 = time.perf_counter()
     ms_plain = (t1 - t0) * 1e3 / ps
 
-    # roofline for the stage with the most device time
-    dev = {k: v for k, v in stages.items() if not k.startswith("setup")}
-    dom = max(dev, key=dev.get)
-    dom_bytes = stage_bytes(stats, dom)
-    achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9 if stages[dom] > 0 else 0.0
     out = {
         "metric": METRIC,
         "value": round(256 / (ms_batch * 1e-3), 2),
@@ -215,16 +228,12 @@ def run_batch_workload(args):
         "plain_meshes_per_s": round(256 / (ms_plain * 1e-3), 2),
         "plain_ms_per_pass_of_256": round(ms_plain, 3),
         "batched_vs_plain": round(ms_plain / ms_batch, 3),
-        "sizes": {"voxels": nv, "nodes": nn, "hits": stats["n_hits"],
-                  "vertices": stats["n_vertices"], "edges": stats["n_edges"],
-                  "out_gb": round(stats["out_bytes"] / 1e9, 3)},
+        "sizes": dict({"voxels": nv, "nodes": nn, "hits": stats["n_hits"]}, **sizes),
+        "passes": ctx.get_option("batch_lanes"),
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "dedup_gbs": dedup_gbs(stats, stages),
         "dedup_path": "bitmap" if stats["bitmap_bytes"] else "sort",
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes": dom_bytes},
+        "roofline": _roofline(stats, stages),
     }
     for dm in dms:
         dm.free()
@@ -238,13 +247,18 @@ def run_batch_workload(args):
 
 
 def _roofline(stats, stages):
+    """Roofline of the stage with the most device time in the last pass:
+    algorithmic bytes of its launch over its HIP-event time on the pass's
+    stream; traffic = measured bytes per algorithmic byte x this launch."""
     dev = {k: v for k, v in stages.items() if not k.startswith("setup") and k != "start"}
     dom = max(dev, key=dev.get)
     b = stage_bytes(stats, dom)
     ach = b / (stages[dom] * 1e-3) / 1e9 if stages[dom] > 0 else 0.0
+    ratio = TRAFFIC_PER_ALG_BYTE.get(dom)
     return {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "algorithmic_bytes": b}
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": round(ratio * b) if ratio else None, "algorithmic_bytes": b,
+            "stage_ms": round(stages[dom], 4)}
 
 
 def run_single_workload(args, name, P, T, res, key_bits, flags=0, desc=""):
@@ -415,6 +429,8 @@ def main():
     ap.add_argument("--max-batch", type=int, default=32)
     ap.add_argument("--max-wait-us", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-lanes", type=int, default=0,
+                    help="concurrent passes a batch is split into (0: library default)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first copy this many MiB device-to-device once: a kernel of known "
                          "traffic for scaling HBM counters")

@@ -122,6 +122,8 @@ const OptionRef kOptions[] = {
     {"bitmap_max_bytes", 0, INT64_MAX},
     {"bitmap_bytes_per_cell", 0, 1 << 20},
     {"dedup_path", 0, 2},
+    {"batch_lanes", 1, 8},
+    {"batch_min_meshes", 1, 1 << 20},
 };
 }  // namespace
 
@@ -139,6 +141,8 @@ int svo_ctx_set_option(svo_ctx* ctx, const char* name, int64_t value) {
     else if (n == "scan_blocks") t.scan_blocks = (uint32_t)value;
     else if (n == "bitmap_max_bytes") t.bitmap_max_bytes = (uint64_t)value;
     else if (n == "bitmap_bytes_per_cell") t.bitmap_bytes_per_cell = (uint32_t)value;
+    else if (n == "batch_lanes") t.batch_lanes = (uint32_t)value;
+    else if (n == "batch_min_meshes") t.batch_min_meshes = (uint32_t)value;
     else t.force_path = (int)value;
     return SVO_OK;
   }
@@ -157,6 +161,8 @@ int svo_ctx_get_option(svo_ctx* ctx, const char* name, int64_t* value) {
   else if (n == "bitmap_max_bytes") *value = (int64_t)t.bitmap_max_bytes;
   else if (n == "bitmap_bytes_per_cell") *value = t.bitmap_bytes_per_cell;
   else if (n == "dedup_path") *value = t.force_path;
+  else if (n == "batch_lanes") *value = t.batch_lanes;
+  else if (n == "batch_min_meshes") *value = t.batch_min_meshes;
   else return SVO_E_INVALID;
   return SVO_OK;
 }

@@ -376,9 +376,15 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_
   __shared__ uint64_t tmp[4];
   __shared__ uint32_t s_word[BM_TILE], s_upre[BM_TILE], s_gpre[BM_TILE];
   __shared__ uint32_t s_mesh;
+  __shared__ uint64_t s_moff[2];  // word range [s_moff[0], s_moff[1]) of mesh s_mesh
   const uint64_t tw0 = (uint64_t)blockIdx.x * BM_TILE;
   const uint32_t l0 = threadIdx.x * BM_WPT;
-  if (threadIdx.x == 0) s_mesh = mesh_of_word(bm_off, n_meshes, tw0);
+  if (threadIdx.x == 0) {
+    const uint32_t m = mesh_of_word(bm_off, n_meshes, tw0);
+    s_mesh = m;
+    s_moff[0] = bm_off[m];
+    s_moff[1] = bm_off[m + 1];
+  }
   uint32_t w[BM_WPT];
   load_words(bits, n_words, tw0 + l0, w);
   uint32_t u = 0, g = 0;
@@ -404,6 +410,19 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_
   __syncthreads();
   const uint32_t bu = tile_u[blockIdx.x], bg = tile_g[blockIdx.x];
   const uint32_t m0 = s_mesh;
+  const uint64_t moff0 = s_moff[0], moff1 = s_moff[1];
+  // mesh of word gw and its first word: the tile's first mesh unless gw lies
+  // past it (a tile spanning several small meshes)
+  auto mesh_at = [&](uint64_t gw, uint32_t& m, uint64_t& off) {
+    if (gw < moff1) {
+      m = m0;
+      off = moff0;
+      return;
+    }
+    m = m0 + 1;
+    while (m + 1 < n_meshes && gw >= bm_off[m + 1]) ++m;
+    off = bm_off[m];
+  };
   const uint32_t nw = (uint32_t)((n_words - tw0) < BM_TILE ? (n_words - tw0) : BM_TILE);
   // unique keys and their groups
   for (uint32_t o = threadIdx.x; o < tu; o += BLOCK) {
@@ -411,9 +430,10 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_
     const uint32_t word = s_word[i];
     const uint32_t b = select_bit(word, o - s_upre[i]);
     const uint64_t gw = tw0 + i;
-    uint32_t m = m0;
-    while (m + 1 < n_meshes && gw >= bm_off[m + 1]) ++m;
-    const K key = ((K)m << kl.cb) | ((K)(gw - bm_off[m]) << 5) | (K)b;
+    uint32_t m;
+    uint64_t off;
+    mesh_at(gw, m, off);
+    const K key = ((K)m << kl.cb) | ((K)(gw - off) << 5) | (K)b;
     elem1[(uint64_t)bu + o] = key;
     group1[(uint64_t)bu + o] = bg + s_gpre[i] + nonzero_bytes_upto(word, b >> 3) - 1;
   }
@@ -430,9 +450,10 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_
     }
     const uint32_t b = 8 * y + (uint32_t)__builtin_ctz((word >> (8 * y)) & 0xFFu);
     const uint64_t gw = tw0 + i;
-    uint32_t m = m0;
-    while (m + 1 < n_meshes && gw >= bm_off[m + 1]) ++m;
-    const K key = ((K)m << kl.cb) | ((K)(gw - bm_off[m]) << 5) | (K)b;
+    uint32_t m;
+    uint64_t off;
+    mesh_at(gw, m, off);
+    const K key = ((K)m << kl.cb) | ((K)(gw - off) << 5) | (K)b;
     elem2[(uint64_t)bg + q] = kl.parent(key);
     first_child2[(uint64_t)bg + q] = bu + s_upre[i] + __popc(word & ((1u << b) - 1u));
   }

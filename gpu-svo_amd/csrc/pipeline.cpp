@@ -22,6 +22,8 @@
 #include <cstring>
 #include <map>
 
+#include <thread>
+
 #include "svo_host.h"
 
 namespace svo {
@@ -280,6 +282,67 @@ uint64_t estimate_bytes(uint64_t n_points, uint64_t n_tris, const svo_params* p)
 }
 
 // ------------------------------------------------------------------ timing
+// Figures of one svo_build* call, which may run as several concurrent
+// passes: each stage's window runs from its earliest start to its latest end
+// over the passes (times from a reference event recorded when the call
+// began), and the pass sizes are summed, so that bytes / window is the
+// stage's aggregate rate.
+struct CallTiming {
+  hipEvent_t ref = nullptr;
+  std::mutex mu;
+  std::vector<std::string> names;
+  std::vector<double> t0, t1;  // per stage window, ms after ref
+  svo_pass_stats ps{};
+  bool any = false;
+  void add(const std::vector<std::string>& nm, const std::vector<double>& a,
+           const std::vector<double>& b, const svo_pass_stats& p) {
+    std::lock_guard<std::mutex> lk(mu);
+    for (size_t i = 0; i < nm.size(); ++i) {
+      size_t k = 0;
+      while (k < names.size() && names[k] != nm[i]) ++k;
+      if (k == names.size()) {
+        names.push_back(nm[i]);
+        t0.push_back(a[i]);
+        t1.push_back(b[i]);
+      } else {
+        t0[k] = std::min(t0[k], a[i]);
+        t1[k] = std::max(t1[k], b[i]);
+      }
+    }
+    if (!any) {
+      ps = p;
+    } else {
+      ps.n_meshes += p.n_meshes;
+      ps.n_tris += p.n_tris;
+      ps.box_cells += p.box_cells;
+      ps.n_hits += p.n_hits;
+      ps.n_voxels += p.n_voxels;
+      ps.level_elems += p.level_elems;
+      ps.n_nodes += p.n_nodes;
+      ps.n_vertices += p.n_vertices;
+      ps.n_edges += p.n_edges;
+      ps.n_faces += p.n_faces;
+      ps.out_bytes += p.out_bytes;
+      ps.bitmap_bytes += p.bitmap_bytes;
+      ps.key_bytes = std::max(ps.key_bytes, p.key_bytes);
+      ps.sort_bits = std::max(ps.sort_bits, p.sort_bits);
+      ps.sort_passes = std::max(ps.sort_passes, p.sort_passes);
+      ps.n_levels = std::max(ps.n_levels, p.n_levels);
+    }
+    any = true;
+  }
+  void publish(Context& C) {
+    if (!any) return;
+    std::lock_guard<std::mutex> lk(C.stage_mu);
+    C.last_pass = ps;
+    if (C.timing && !names.empty()) {
+      C.stage_names = names;
+      C.stage_ms.resize(names.size());
+      for (size_t k = 0; k < names.size(); ++k) C.stage_ms[k] = t1[k] - t0[k];
+    }
+  }
+};
+
 struct StageTimer {
   Context& C;
   Lane& L;
@@ -297,19 +360,20 @@ struct StageTimer {
     names.push_back(name);
     ++used;
   }
-  void collect() {
-    if (!C.timing || used < 2) return;
-    std::vector<double> ms;
+  // Hands the pass's stage windows and sizes to the call's figures.
+  void publish(CallTiming& ct, const svo_pass_stats& ps) {
+    std::vector<double> a, b;
     std::vector<std::string> nm;
-    for (size_t i = 1; i < used; ++i) {
-      float t = 0;
-      hipEventElapsedTime(&t, L.events[i - 1], L.events[i]);
-      ms.push_back(t);
-      nm.push_back(names[i]);
-    }
-    std::lock_guard<std::mutex> lk(C.stage_mu);
-    C.stage_ms = ms;
-    C.stage_names = nm;
+    if (C.timing && ct.ref && used >= 2)
+      for (size_t i = 1; i < used; ++i) {
+        float x = 0, y = 0;
+        hipEventElapsedTime(&x, ct.ref, L.events[i - 1]);
+        hipEventElapsedTime(&y, ct.ref, L.events[i]);
+        a.push_back(x);
+        b.push_back(y);
+        nm.push_back(names[i]);
+      }
+    ct.add(nm, a, b, ps);
   }
 };
 
@@ -345,6 +409,7 @@ bool no_adjacent_duplicates(const std::vector<MeshNodeLayout>& lay, const MeshPa
 }
 
 struct Plan {
+  CallTiming* timing = nullptr;
   std::vector<uint32_t> idx;  // active mesh -> input index
   int bits = 1;               // per-axis coordinate bits of the batch
   int cb = 3;                 // code bits
@@ -447,7 +512,15 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   // ---- A0: bbox, params, classification
   launch_bbox(d_md, n, max_points, d_acc, s);
   launch_params(d_md, n, d_acc, d_par, s);
-  launch_classify(d_md, d_par, d_tb, n, T, C.tuning, d_cnt, s);
+  // Triangles with at most small_max_cells box cells are voxelized one lane
+  // each, larger ones by whole waves over 256-row items.  A pass with few
+  // triangles has too few lanes to hide a long per-lane loop, so its limit
+  // shrinks with the triangle count (32 cells below 4K triangles, the full
+  // limit from 64K triangles).
+  Tuning tn = C.tuning;
+  tn.small_max_cells = (uint32_t)std::min<uint64_t>(
+      tn.small_max_cells, std::max<uint64_t>(32, (uint64_t)tn.small_max_cells * T / 65536));
+  launch_classify(d_md, d_par, d_tb, n, T, tn, d_cnt, s);
   timer.mark("classify");
   HIP_TRY(hipMemcpyAsync(L.pinned, d_cnt, 16, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -540,7 +613,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     launch_fill_u32(bitmap, 0u, bm_words, s);
     timer.mark("bitmap_clear");
   }
-  launch_voxelize<K>(d_md, d_par, d_tb, n, T, plan.cb, C.tuning, keys_a, bitmap, d_bo,
+  launch_voxelize<K>(d_md, d_par, d_tb, n, T, plan.cb, tn, keys_a, bitmap, d_bo,
                      d_cnt + 2, big, d_cnt + 3, n_big, s);
   timer.mark("voxelize");
   if (use_bm) {
@@ -718,7 +791,6 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   HIP_TRY(hipMemcpyAsync(h_tot, d_tot, 8 * (4 + 3 * n), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   HIP_TRY(hipGetLastError());
-  timer.collect();
 
   // ---- trees
   std::vector<uint64_t> tot(h_tot, h_tot + 4 + 3 * n);
@@ -740,8 +812,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     ps.sort_passes = use_bm ? 0 : (ps.sort_bits + 7) / 8;
     ps.bitmap_bytes = use_bm ? bm_bytes : 0;
     ps.n_levels = NL;
-    std::lock_guard<std::mutex> lk(C.stage_mu);
-    C.last_pass = ps;
+    timer.publish(*plan.timing, ps);
   }
   for (uint32_t i = 0; i < n; ++i) {
     const uint32_t src = plan.idx[i];
@@ -801,31 +872,29 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
 
 }  // namespace
 
-int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree*>& trees,
-              std::vector<int>& statuses) {
-  trees.assign(in.size(), nullptr);
-  statuses.assign(in.size(), SVO_OK);
+namespace {
+
+// One pipeline pass over the valid meshes `sel` of `in` on one lane.
+int run_pass(Context& C, const std::vector<MeshInput>& in, const std::vector<uint32_t>& sel,
+             std::vector<svo_tree*>& trees, std::vector<int>& statuses, CallTiming& ct) {
   Plan plan;
+  plan.timing = &ct;
   bool any64 = false;
-  for (uint32_t i = 0; i < in.size(); ++i) {
-    int kb = 32;
-    const int st = validate_params(&in[i].desc, &in[i].params, &kb);
-    statuses[i] = st;
-    if (st != SVO_OK) continue;
-    if (!plan.idx.empty() && (in[i].params.semantics != in[plan.idx[0]].params.semantics ||
-                              in[i].params.flags != in[plan.idx[0]].params.flags)) {
-      statuses[i] = SVO_E_INVALID;  // one semantics / flag set per pass
-      continue;
-    }
+  for (uint32_t i : sel) {
     plan.idx.push_back(i);
-    any64 |= kb == 64;
+    any64 |= in[i].params.key_bits == 64 ||
+             (in[i].params.key_bits == 0 &&
+              std::max({in[i].params.res[0], in[i].params.res[1], in[i].params.res[2]}) > 1024);
     for (int a = 0; a < 3; ++a) plan.bits = std::max(plan.bits, bits_for(in[i].params.res[a]));
   }
   if (plan.idx.empty()) return SVO_OK;
   plan.cb = 3 * plan.bits;
   plan.mb = bits_for((int32_t)plan.idx.size());
   plan.n_levels = plan.bits + 1;
-  if (plan.cb + plan.mb > 64) return SVO_E_RES_LIMIT;
+  if (plan.cb + plan.mb > 64) {
+    for (uint32_t i : plan.idx) statuses[i] = SVO_E_RES_LIMIT;
+    return SVO_E_RES_LIMIT;
+  }
   plan.k64 = plan.cb + plan.mb > 32;
   plan.code_bytes = any64 ? 8 : 4;
   Lane* L = C.acquire_lane();
@@ -836,14 +905,97 @@ int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree
   if (st != SVO_OK) {
     hipStreamSynchronize(L->stream);
     hipGetLastError();
-    for (auto& t : trees) {
-      delete t;
-      t = nullptr;
+    for (uint32_t i : plan.idx) {
+      delete trees[i];
+      trees[i] = nullptr;
+      statuses[i] = st;
     }
-    for (uint32_t i : plan.idx) statuses[i] = st;
   }
   C.release_lane(L);
   return st;
+}
+
+}  // namespace
+
+int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree*>& trees,
+              std::vector<int>& statuses) {
+  trees.assign(in.size(), nullptr);
+  statuses.assign(in.size(), SVO_OK);
+  std::vector<uint32_t> valid;
+  for (uint32_t i = 0; i < in.size(); ++i) {
+    int kb = 32;
+    const int st = validate_params(&in[i].desc, &in[i].params, &kb);
+    statuses[i] = st;
+    if (st != SVO_OK) continue;
+    if (!valid.empty() && (in[i].params.semantics != in[valid[0]].params.semantics ||
+                           in[i].params.flags != in[valid[0]].params.flags)) {
+      statuses[i] = SVO_E_INVALID;  // one semantics / flag set per pass
+      continue;
+    }
+    valid.push_back(i);
+  }
+  if (valid.empty()) return SVO_OK;
+  CallTiming ct;
+  if (C.timing) {
+    hipSetDevice(C.device);
+    if (hipEventCreate(&ct.ref) == hipSuccess) {
+      // on the null stream, which the lanes' non-blocking streams do not
+      // wait for and nothing else uses: it completes when the call begins
+      if (hipEventRecord(ct.ref, nullptr) != hipSuccess) {
+        hipEventDestroy(ct.ref);
+        ct.ref = nullptr;
+      }
+    } else {
+      ct.ref = nullptr;
+    }
+  }
+  struct RefGuard {
+    CallTiming& ct;
+    Context& C;
+    ~RefGuard() {
+      ct.publish(C);
+      if (ct.ref) hipEventDestroy(ct.ref);
+    }
+  } ref_guard{ct, C};
+  // Large batches are split into sub-batches that run as concurrent passes on
+  // separate lanes (streams): their host round trips and kernel tails
+  // overlap.
+  // Meshes go to the lightest sub-batch, heaviest first; the weight estimates
+  // triangles plus surface voxels (~ res^2).
+  uint32_t groups = 1;
+  {
+    Tuning t;
+    {
+      std::lock_guard<std::mutex> lk(C.mu);
+      t = C.tuning;
+    }
+    if (t.batch_lanes > 1 && valid.size() >= 2 * (size_t)t.batch_min_meshes)
+      groups = std::min<uint32_t>(t.batch_lanes, (uint32_t)(valid.size() / t.batch_min_meshes));
+  }
+  if (groups <= 1) return run_pass(C, in, valid, trees, statuses, ct);
+  std::vector<std::pair<double, uint32_t>> w;
+  for (uint32_t i : valid) {
+    const double r = std::max({in[i].params.res[0], in[i].params.res[1], in[i].params.res[2]});
+    w.push_back({3.0 * (double)in[i].desc.n_tris + 10.0 * r * r, i});
+  }
+  std::sort(w.begin(), w.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  std::vector<std::vector<uint32_t>> sel(groups);
+  std::vector<double> load(groups, 0.0);
+  for (const auto& e : w) {
+    const uint32_t g = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+    sel[g].push_back(e.second);
+    load[g] += e.first;
+  }
+  for (auto& v : sel) std::sort(v.begin(), v.end());
+  std::vector<int> rc(groups, SVO_OK);
+  std::vector<std::thread> th;
+  for (uint32_t g = 1; g < groups; ++g)
+    th.emplace_back([&, g] { rc[g] = run_pass(C, in, sel[g], trees, statuses, ct); });
+  rc[0] = run_pass(C, in, sel[0], trees, statuses, ct);
+  for (auto& t : th) t.join();
+  for (int r : rc)
+    if (r != SVO_OK) return r;
+  return SVO_OK;
 }
 
 }  // namespace svo

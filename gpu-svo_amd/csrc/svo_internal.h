@@ -21,6 +21,10 @@ struct Tuning {
   uint64_t bitmap_max_bytes = 32ull << 30;
   uint32_t bitmap_bytes_per_cell = 64;
   int force_path = 0;               // 0 automatic, 1 bitmap, 2 sort
+  // svo_build_batch runs batches of at least 2 * batch_min_meshes meshes as
+  // up to batch_lanes concurrent passes
+  uint32_t batch_lanes = 2;
+  uint32_t batch_min_meshes = 16;
 };
 
 // ------------------------------------------------------------ k_voxelize.hip

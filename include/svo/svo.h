@@ -147,8 +147,10 @@ typedef struct {
  * bound beyond the device). */
 int svo_ctx_create(int device, uint64_t mem_budget, svo_ctx** out);
 int svo_ctx_destroy(svo_ctx* ctx);
-/* Per-stage device times of the last build on this context, in milliseconds,
- * measured with HIP events on the build's stream when timing is enabled. */
+/* Per-stage device times of the last build call on this context, in
+ * milliseconds, measured with HIP events on the builds' streams when timing
+ * is enabled.  A call that runs as several concurrent passes reports, per
+ * stage, the window from the stage's earliest start to its latest end. */
 int svo_ctx_set_timing(svo_ctx* ctx, int enable);
 int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap, int* n);
 /* Tuning options of the context (no counterpart in the reference), set
@@ -159,10 +161,12 @@ int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap,
  *   "bitmap_max_bytes"       largest occupancy bitmap used instead of the sort (32 GiB)
  *   "bitmap_bytes_per_cell"  bitmap bytes allowed per candidate cell (64)
  *   "dedup_path"             0 automatic, 1 always the bitmap, 2 always the sort
+ *   "batch_lanes"            concurrent passes a large svo_build_batch is split into (2)
+ *   "batch_min_meshes"       fewest meshes per such pass (16)
  * Unknown names or out-of-range values give SVO_E_INVALID. */
 int svo_ctx_set_option(svo_ctx* ctx, const char* name, int64_t value);
 int svo_ctx_get_option(svo_ctx* ctx, const char* name, int64_t* value);
-/* Sizes of the last pipeline pass on this context (all meshes of the pass). */
+/* Sizes of the last build call on this context, summed over its passes. */
 typedef struct {
   uint64_t n_meshes, n_tris, box_cells, n_hits, n_voxels, level_elems, n_nodes;
   uint64_t n_vertices, n_edges, n_faces, out_bytes;
