@@ -108,13 +108,14 @@ __device__ inline bool claim_tile(const K* src, uint64_t n, uint32_t* ticket, ui
 }
 
 // ---------------------------------------------------------------- level 1
-// Sorted hit keys -> unique keys (elem1) with their group (group1), the level-2
-// keys (elem2 = group parents) with their first child (first_child2), the
-// per-mesh spans of level 1, and counts[0] / counts[1].
+// Sorted hit keys -> unique keys (elem1), the level-2 keys (elem2 = group
+// parents) with their first child (first_child2), the per-mesh spans of
+// level 1, and counts[0] / counts[1].  (Level 1's own group index per key is
+// not needed: node records look groups up from the level above.)
 template <typename K>
 __global__ __launch_bounds__(BLOCK) void k_first_pass(
-    const K* sorted, const unsigned long long* n_ptr, KeyLayout<K> kl, K* elem1,
-    uint32_t* group1, K* elem2, uint32_t* first_child2, LevelSpan* spans, uint32_t n_levels,
+    const K* sorted, const unsigned long long* n_ptr, KeyLayout<K> kl, K* elem1, K* elem2,
+    uint32_t* first_child2, LevelSpan* spans, uint32_t n_levels,
     uint64_t* status, uint32_t* ticket, uint32_t epoch, unsigned long long* counts) {
   __shared__ K s_edge[2][BLOCK];
   __shared__ K s_key[TILE];
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(BLOCK) void k_first_pass(
     }
     __syncthreads();
     const uint32_t bu = s_base[0], bg = s_base[1];
-    // elem1 / group1 staged by tile-local unique index; spans on the way
+    // elem1 staged by tile-local unique index; spans on the way
     {
       uint32_t ru = (uint32_t)ex, rg = (uint32_t)(ex >> 32);
       K pk = r.prev;
@@ -166,7 +167,6 @@ __global__ __launch_bounds__(BLOCK) void k_first_pass(
           if (fl & 1u) {
             if (fl & 2u) ++rg;
             s_key[ru] = key;
-            s_val[ru] = bg + rg - 1;
             if (!hp || kl.mesh(pk) != m) {  // first element of its mesh
               spans[(uint64_t)m * n_levels].elem_first = bu + ru;
               spans[(uint64_t)m * n_levels].group_first = bg + rg - 1;
@@ -185,10 +185,7 @@ __global__ __launch_bounds__(BLOCK) void k_first_pass(
       }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < tu; i += BLOCK) {
-      elem1[(uint64_t)bu + i] = s_key[i];
-      group1[(uint64_t)bu + i] = s_val[i];
-    }
+    for (uint32_t i = threadIdx.x; i < tu; i += BLOCK) elem1[(uint64_t)bu + i] = s_key[i];
     __syncthreads();
     // elem2 / first_child2 staged by tile-local group index
     {
@@ -223,7 +220,15 @@ __global__ __launch_bounds__(BLOCK) void k_first_pass(
 // are the parent groups (the eight children of parent p are codes 8p..8p+7,
 // one byte).  count -> scan of tile counts -> emit, each over tiles of 2048
 // words.
+constexpr int WAVES_LV = BLOCK / 64;
 constexpr int BM_WPT = 8;                 // words per thread (two 16-byte loads)
+
+// Orders LDS accesses of different lanes of one wave.
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 constexpr int BM_TILE = BLOCK * BM_WPT;   // 2048 words
 
 __device__ inline uint32_t nonzero_bytes(uint32_t w) {
@@ -345,36 +350,21 @@ __device__ inline uint32_t select_bit(uint32_t w, uint32_t k) {
   return pos;
 }
 
-// Nonzero bytes of w among bytes 0 .. y.
-__device__ inline uint32_t nonzero_bytes_upto(uint32_t w, uint32_t y) {
-  return nonzero_bytes(y >= 3 ? w : (w & ((1u << (8 * y + 8)) - 1u)));
-}
-
-// Index of the last entry of the sorted array a[0 .. n) that is <= v
-// (a[0] = 0 <= v), by binary search in LDS.
-__device__ inline uint32_t last_le(const uint32_t* a, uint32_t n, uint32_t v) {
-  uint32_t lo = 0, hi = n;  // a[lo] <= v < a[hi] (a[n] taken as infinite)
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (a[mid] <= v) lo = mid; else hi = mid;
-  }
-  return lo;
-}
 
 // One tile of 2048 words.  The tile's words and the exclusive prefixes of
-// their set bits and nonzero bytes go to LDS; then every output is produced
-// by one thread, in output order, so the stores are contiguous: unique key o
-// is the k-th set bit of the word whose bit prefix holds o, group g the k-th
-// nonzero byte of the word whose byte prefix holds g.
+// their set bits and nonzero bytes go to LDS; then the outputs are produced
+// in output order so that the stores are contiguous: unique key o is the
+// k-th set bit of the word whose bit prefix holds o, group g the k-th nonzero
+// byte of the word whose byte prefix holds g.
 template <typename K>
 __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_t n_words,
                                                    const uint64_t* bm_off, uint32_t n_meshes,
                                                    KeyLayout<K> kl, const uint32_t* tile_u,
-                                                   const uint32_t* tile_g, K* elem1,
-                                                   uint32_t* group1, K* elem2,
+                                                   const uint32_t* tile_g, K* elem1, K* elem2,
                                                    uint32_t* first_child2) {
   __shared__ uint64_t tmp[4];
   __shared__ uint32_t s_word[BM_TILE], s_upre[BM_TILE], s_gpre[BM_TILE];
+  __shared__ int32_t s_own[WAVES_LV][64];
   __shared__ uint32_t s_mesh;
   __shared__ uint64_t s_moff[2];  // word range [s_moff[0], s_moff[1]) of mesh s_mesh
   const uint64_t tw0 = (uint64_t)blockIdx.x * BM_TILE;
@@ -395,7 +385,6 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_
   }
   uint64_t tot;
   const uint64_t ex = block_excl_scan256(((uint64_t)g << 32) | u, tmp, tot);
-  const uint32_t tu = (uint32_t)tot, tg = (uint32_t)(tot >> 32);
   {
     uint32_t pu = (uint32_t)ex, pg = (uint32_t)(ex >> 32);
 #pragma unroll
@@ -423,39 +412,66 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_
     while (m + 1 < n_meshes && gw >= bm_off[m + 1]) ++m;
     off = bm_off[m];
   };
-  const uint32_t nw = (uint32_t)((n_words - tw0) < BM_TILE ? (n_words - tw0) : BM_TILE);
-  // unique keys and their groups
-  for (uint32_t o = threadIdx.x; o < tu; o += BLOCK) {
-    const uint32_t i = last_le(s_upre, nw, o);
-    const uint32_t word = s_word[i];
-    const uint32_t b = select_bit(word, o - s_upre[i]);
-    const uint64_t gw = tw0 + i;
+  // Each wave emits the outputs of its quarter of the tile, 64 words at a
+  // time.  Lane j holds word i0 + j and the tile prefixes of its set bits
+  // (pu) and nonzero bytes (pg).  The window's outputs go out 64 per round:
+  // a lane whose range starts inside the round marks that slot with its lane
+  // id, a max-scan over the slots gives every output slot its word, and the
+  // output is the k-th set bit (or nonzero byte) of that word.  Consecutive
+  // lanes store consecutive outputs.
+  const uint32_t wv = threadIdx.x >> 6, lane = __lane_id();
+  int32_t* own = s_own[wv];
+  for (uint32_t i0 = wv * (BM_TILE / WAVES_LV); i0 < (wv + 1) * (BM_TILE / WAVES_LV); i0 += 64) {
+    const uint32_t idx = i0 + lane;
+    const uint32_t word = s_word[idx];
+    const uint32_t pu = s_upre[idx], pg = s_gpre[idx];
+    const uint32_t cu = __popc(word), cg = nonzero_bytes(word);
     uint32_t m;
     uint64_t off;
-    mesh_at(gw, m, off);
-    const K key = ((K)m << kl.cb) | ((K)(gw - off) << 5) | (K)b;
-    elem1[(uint64_t)bu + o] = key;
-    group1[(uint64_t)bu + o] = bg + s_gpre[i] + nonzero_bytes_upto(word, b >> 3) - 1;
-  }
-  // groups: parent key and first child (the lowest set bit of the byte)
-  for (uint32_t q = threadIdx.x; q < tg; q += BLOCK) {
-    const uint32_t i = last_le(s_gpre, nw, q);
-    const uint32_t word = s_word[i];
-    uint32_t k = q - s_gpre[i], y = 0;
-    for (; y < 4; ++y) {
-      if ((word >> (8 * y)) & 0xFFu) {
-        if (k == 0) break;
-        --k;
+    mesh_at(tw0 + idx, m, off);
+    const K hi = ((K)m << kl.cb) | ((K)(tw0 + idx - off) << 5);
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+      // pass 0: unique keys (set bits); pass 1: groups (nonzero bytes)
+      const uint32_t p = pass == 0 ? pu : pg, c = pass == 0 ? cu : cg;
+      const uint32_t start = __shfl(p, 0), end = __shfl(p + c, 63);
+      for (uint32_t base = start; base < end; base += 64) {
+        own[lane] = 0;
+        wave_sync();
+        if (c > 0) {
+          const uint32_t first = p > base ? p : base;
+          if (first < p + c && first < base + 64) own[first - base] = (int32_t)lane;
+        }
+        wave_sync();
+        int32_t j = own[lane];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+          const int32_t t = __shfl_up(j, d);
+          if (lane >= (uint32_t)d && t > j) j = t;
+        }
+        wave_sync();
+        const uint32_t o = base + lane;
+        const uint32_t wj = __shfl(word, j), pj = __shfl(p, j);
+        const K hj = __shfl(hi, j);
+        if (o < end) {
+          if (pass == 0) {
+            elem1[(uint64_t)bu + o] = hj | (K)select_bit(wj, o - pj);
+          } else {
+            uint32_t k = o - pj, y = 0;
+            for (; y < 4; ++y) {
+              if ((wj >> (8 * y)) & 0xFFu) {
+                if (k == 0) break;
+                --k;
+              }
+            }
+            const uint32_t b = 8 * y + (uint32_t)__builtin_ctz((wj >> (8 * y)) & 0xFFu);
+            const uint32_t uj = __shfl(pu, j);
+            elem2[(uint64_t)bg + o] = kl.parent(hj | (K)b);
+            first_child2[(uint64_t)bg + o] = bu + uj + __popc(wj & ((1u << b) - 1u));
+          }
+        }
       }
     }
-    const uint32_t b = 8 * y + (uint32_t)__builtin_ctz((word >> (8 * y)) & 0xFFu);
-    const uint64_t gw = tw0 + i;
-    uint32_t m;
-    uint64_t off;
-    mesh_at(gw, m, off);
-    const K key = ((K)m << kl.cb) | ((K)(gw - off) << 5) | (K)b;
-    elem2[(uint64_t)bg + q] = kl.parent(key);
-    first_child2[(uint64_t)bg + q] = bu + s_upre[i] + __popc(word & ((1u << b) - 1u));
   }
 }
 
@@ -663,7 +679,7 @@ void launch_level_first(const K* sorted, const unsigned long long* n_sorted, uin
   hipLaunchKernelGGL(k_level_clear, dim3(std::max<uint32_t>(cg, 1)), dim3(BLOCK), 0, s, status,
                      tickets, n_sorted);
   hipLaunchKernelGGL(k_first_pass<K>, dim3(grid_tiles(cap, blocks)), dim3(BLOCK), 0, s, sorted,
-                     n_sorted, kl, (K*)l1.elem, l1.elem_group, (K*)l2.elem, l2.first_child,
+                     n_sorted, kl, (K*)l1.elem, (K*)l2.elem, l2.first_child,
                      spans1, n_levels, status, tickets, 1u, counts);
 }
 
@@ -691,7 +707,7 @@ void launch_level_first_bitmap(const uint32_t* bits, uint64_t n_words, const uin
                      (uint32_t)tiles, scan_status, counts);
   hipLaunchKernelGGL(k_bm_emit<K>, dim3((uint32_t)tiles), dim3(BLOCK), 0, s, bits, n_words,
                      bm_off, n_meshes, kl, (const uint32_t*)tile_u, (const uint32_t*)tile_g,
-                     (K*)l1.elem, l1.elem_group, (K*)l2.elem, l2.first_child);
+                     (K*)l1.elem, (K*)l2.elem, l2.first_child);
   hipLaunchKernelGGL(k_bm_spans<K>, dim3(n_meshes), dim3(BLOCK), 0, s,
                      (const K*)l1.elem, (const K*)l2.elem, (const unsigned long long*)counts, kl,
                      n_meshes, spans1, n_levels);

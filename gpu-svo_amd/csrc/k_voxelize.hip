@@ -238,12 +238,14 @@ struct RowTerms {
   float rxy[3];  // ne_xy[i].y * p.y
   float rzx[3];  // ne_zx[i].x * p.z
 };
+// The tests are combined without short-circuit: every lane evaluates all of
+// them (the same operations and roundings), so the wave runs no branches.
 __device__ inline bool row_terms(const TriSetup& S, int y, int z, RowTerms& R) {
   const float py = (float)y * S.u[1];
   const float pz = (float)z * S.u[2];
   bool ok = true;
   for (int i = 0; i < 3; ++i)
-    ok = ok && !((S.ne[1][i][0] * py + S.ne[1][i][1] * pz) + S.de[1][i] < 0.0f);
+    ok &= !((S.ne[1][i][0] * py + S.ne[1][i][1] * pz) + S.de[1][i] < 0.0f);
   R.A = S.n[1] * py + S.n[2] * pz;
   for (int i = 0; i < 3; ++i) {
     R.rxy[i] = S.ne[0][i][1] * py;
@@ -254,11 +256,10 @@ __device__ inline bool row_terms(const TriSetup& S, int y, int z, RowTerms& R) {
 __device__ inline bool cell_hit(const TriSetup& S, const RowTerms& R, int x) {
   const float px = (float)x * S.u[0];
   const float t = S.n[0] * px + R.A;
-  if ((t + S.d1) * (t + S.d2) > 0.0f) return false;
-  bool ok = true;
+  bool ok = !((t + S.d1) * (t + S.d2) > 0.0f);
   for (int i = 0; i < 3; ++i) {
-    ok = ok && !((S.ne[0][i][0] * px + R.rxy[i]) + S.de[0][i] < 0.0f);
-    ok = ok && !((R.rzx[i] + S.ne[2][i][1] * px) + S.de[2][i] < 0.0f);
+    ok &= !((S.ne[0][i][0] * px + R.rxy[i]) + S.de[0][i] < 0.0f);
+    ok &= !((R.rzx[i] + S.ne[2][i][1] * px) + S.de[2][i] < 0.0f);
   }
   return ok;
 }
@@ -549,7 +550,7 @@ __global__ __launch_bounds__(BLOCK) void k_vox_big(const MeshDev* meshes,
         if (row_terms(S, y, z, R)) row_range(S, R, xa, xb);
       }
       for (int x = xa; __any(x <= xb); ++x) {
-        const bool hit = x <= xb && cell_hit(S, R, x);
+        const bool hit = (x <= xb) & cell_hit(S, R, x);
         const K code = morton3<K>((uint32_t)x, (uint32_t)y, (uint32_t)z);
         if (BITMAP) bs.push(hit, wbase + (uint64_t)(code >> 5), (uint32_t)(code & 31));
         else st.push(hit, mesh_key | code);

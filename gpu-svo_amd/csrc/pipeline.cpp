@@ -324,6 +324,7 @@ struct CallTiming {
       ps.n_faces += p.n_faces;
       ps.out_bytes += p.out_bytes;
       ps.bitmap_bytes += p.bitmap_bytes;
+      ps.n_groups1 += p.n_groups1;
       ps.key_bytes = std::max(ps.key_bytes, p.key_bytes);
       ps.sort_bits = std::max(ps.sort_bits, p.sort_bits);
       ps.sort_passes = std::max(ps.sort_passes, p.sort_passes);
@@ -580,20 +581,22 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   std::vector<LevelArrays> lv(NL + 2);
   {
     size_t total = 0;
+    // level 1 keeps only its keys: its group index per key is never read
     for (uint32_t l = 1; l <= NL + 1; ++l)
-      total += align256(sizeof(K) * cap[l]) + align256(4 * cap[l]) + (l >= 2 ? align256(4 * cap[l]) : 0);
+      total += align256(sizeof(K) * cap[l]) + (l >= 2 ? 2 * align256(4 * cap[l]) : 0);
     char* base = nullptr;
     SVO_TRY(W.get("levels", total, (void**)&base));
     for (uint32_t l = 1; l <= NL + 1; ++l) {
       lv[l].cap = cap[l];
       lv[l].elem = base;
       base += align256(sizeof(K) * cap[l]);
-      lv[l].elem_group = (uint32_t*)base;
-      base += align256(4 * cap[l]);
       if (l >= 2) {
+        lv[l].elem_group = (uint32_t*)base;
+        base += align256(4 * cap[l]);
         lv[l].first_child = (uint32_t*)base;
         base += align256(4 * cap[l]);
       } else {
+        lv[l].elem_group = nullptr;
         lv[l].first_child = nullptr;
       }
     }
@@ -811,6 +814,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     ps.sort_bits = (uint32_t)(plan.cb + plan.mb);
     ps.sort_passes = use_bm ? 0 : (ps.sort_bits + 7) / 8;
     ps.bitmap_bytes = use_bm ? bm_bytes : 0;
+    ps.n_groups1 = h_counts[1];
     ps.n_levels = NL;
     timer.publish(*plan.timing, ps);
   }

@@ -62,7 +62,7 @@ void launch_radix_sort(K* a, K* b, const unsigned long long* n, uint64_t cap, in
 // ------------------------------------------------------------ k_levels.hip
 struct LevelArrays {
   void* elem;            // K[cap]: sorted unique keys of this level (S_l)
-  uint32_t* elem_group;  // [cap]: group index (at this level) of each element
+  uint32_t* elem_group;  // [cap]: group index (at this level) of each element (l >= 2)
   uint32_t* first_child; // [cap]: for groups of the level below: first member index
   uint64_t cap;
 };
