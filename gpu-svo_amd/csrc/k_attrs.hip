@@ -107,72 +107,101 @@ __device__ inline GroupNode<K> group_node(int level, KeyLayout<K> kl, const K* e
   return r;
 }
 
-// One thread per node, eight per parent group.  The loads are issued in two
-// rounds that do not depend on each other within a round (group arrays, then
-// everything keyed by the mesh and the first child), so a wave waits for two
-// memory latencies before it stores.
+// Everything a node record needs, gathered in two rounds of loads that do not
+// depend on each other within a round: the group arrays, then what is keyed
+// by the mesh and by the first child.
+template <typename K>
+struct NodeIn {
+  K P, e;
+  uint32_t fc, gnext, depth, spn_first;
+  uint64_t g, nn, nbase, off_l, off_p, off_c;
+  LevelSpan sp;
+  float w, ox, oy, oz;
+  bool in;
+};
+
+constexpr int NODES_PT = 4;  // nodes per thread, their loads in flight together
+
+// One thread per node, eight per parent group, NODES_PT nodes per thread
+// BLOCK apart (so stores stay contiguous across a wave).  All loads of the
+// thread's nodes are issued before the first is used.
 template <typename K>
 __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
     int level, KeyLayout<K> kl, const K* elem_l, uint64_t n_elem_l, const K* elem_next,
     const uint32_t* group_next, const uint32_t* first_child_next, uint64_t n_groups,
     const LevelSpan* spans, uint32_t n_levels, const MeshTreeState* state,
     const MeshNodeLayout* layout, const MeshParams* params, NodeArrays out) {
-  const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-  const uint64_t g0 = gid >> 3;
-  const uint32_t k = (uint32_t)(gid & 7);
-  const bool in = g0 < n_groups;
-  const uint64_t g = in ? g0 : 0;
-  // round 1: the group's parent key, first child and parent group
-  const K P = elem_next[g];
-  const uint32_t fc = first_child_next[g];
-  const uint32_t gnext = group_next[g];
-  // round 2: mesh state, layout and spans; member k of the group
-  const uint32_t m = kl.mesh(P);
-  const uint32_t depth = state[m].depth;
-  const MeshNodeLayout* L = layout + m;
-  const uint64_t nn = L->n_nodes, nbase = L->node_base;
-  const uint64_t off_l = L->level_off[level - 1], off_p = L->level_off[level];
-  const uint64_t off_c = level >= 2 ? L->level_off[level - 2] : 0;
-  const float w = L->width[level - 1];
-  const LevelSpan sp = spans[(uint64_t)m * n_levels + (level - 1)];
-  const uint32_t spn_first = spans[(uint64_t)m * n_levels + level].group_first;
-  const float ox = params[m].origin[0], oy = params[m].origin[1], oz = params[m].origin[2];
-  const uint64_t j = (uint64_t)fc + k;
-  const K e = j < n_elem_l ? elem_l[j] : (K)0;
-  // a depth-1 tree's level-1 group is no node group; a mesh rejected after the
-  // level phase has no nodes
-  const bool live = in && depth > (uint32_t)level && nn != 0;
-  uint32_t bit = (live && j < n_elem_l && kl.parent(e) == P) ? 1u << (uint32_t)(e & 7) : 0u;
-  bit |= __shfl_xor(bit, 1);
-  bit |= __shfl_xor(bit, 2);
-  bit |= __shfl_xor(bit, 4);
-  if (!live) return;
-  const uint32_t mask = bit;
-  const bool real = (mask >> k) & 1u;
-  const uint64_t local = off_l + 8ull * (g - sp.group_first) + k;
-  const uint64_t t = nbase + local;
-  const uint32_t parent = depth == (uint32_t)level + 1
-                              ? (uint32_t)off_p  // the root
-                              : (uint32_t)(off_p + 8ull * (gnext - spn_first) +
-                                           (uint32_t)(P & 7));
-  const K code = (kl.code(P) << 3) | (K)k;
-  uint32_t x, y, z;
-  demorton3<K>(code, x, y, z);
-  store_code(out, t, (uint64_t)code);
-  out.is_leaf[t] = (level == 1 || !real) ? 1 : 0;
-  // gridOrigin + width * Vector3f(x, y, z), per axis (SVO.cu:262)
-  out.origin[3 * t + 0] = ox + w * (float)x;
-  out.origin[3 * t + 1] = oy + w * (float)y;
-  out.origin[3 * t + 2] = oz + w * (float)z;
-  out.width[t] = w;
-  out.parent[t] = parent;
-  uint32_t cbase = 0;
-  const bool has_childs = real && level >= 2;
-  if (has_childs) {
-    const uint64_t i = (uint64_t)fc + __popc(mask & ((1u << k) - 1u));
-    cbase = (uint32_t)(off_c + 8ull * (i - sp.elem_first));
+  const uint64_t base = (uint64_t)blockIdx.x * BLOCK * NODES_PT + threadIdx.x;
+  const uint32_t k = (uint32_t)(threadIdx.x & 7);
+  NodeIn<K> a[NODES_PT];
+#pragma unroll
+  for (int u = 0; u < NODES_PT; ++u) {  // round 1
+    const uint64_t g0 = (base + (uint64_t)u * BLOCK) >> 3;
+    a[u].in = g0 < n_groups;
+    a[u].g = a[u].in ? g0 : 0;
+    a[u].P = elem_next[a[u].g];
+    a[u].fc = first_child_next[a[u].g];
+    a[u].gnext = group_next[a[u].g];
   }
-  store_childs(out.childs + 8 * t, cbase, has_childs);
+#pragma unroll
+  for (int u = 0; u < NODES_PT; ++u) {  // round 2
+    const uint32_t m = kl.mesh(a[u].P);
+    a[u].depth = state[m].depth;
+    const MeshNodeLayout* L = layout + m;
+    a[u].nn = L->n_nodes;
+    a[u].nbase = L->node_base;
+    a[u].off_l = L->level_off[level - 1];
+    a[u].off_p = L->level_off[level];
+    a[u].off_c = level >= 2 ? L->level_off[level - 2] : 0;
+    a[u].w = L->width[level - 1];
+    a[u].sp = spans[(uint64_t)m * n_levels + (level - 1)];
+    a[u].spn_first = spans[(uint64_t)m * n_levels + level].group_first;
+    a[u].ox = params[m].origin[0];
+    a[u].oy = params[m].origin[1];
+    a[u].oz = params[m].origin[2];
+    const uint64_t j = (uint64_t)a[u].fc + k;
+    a[u].e = j < n_elem_l ? elem_l[j] : (K)0;
+  }
+#pragma unroll
+  for (int u = 0; u < NODES_PT; ++u) {
+    const NodeIn<K>& q = a[u];
+    const uint64_t j = (uint64_t)q.fc + k;
+    // a depth-1 tree's level-1 group is no node group; a mesh rejected after
+    // the level phase has no nodes
+    const bool live = q.in && q.depth > (uint32_t)level && q.nn != 0;
+    uint32_t bit =
+        (live && j < n_elem_l && kl.parent(q.e) == q.P) ? 1u << (uint32_t)(q.e & 7) : 0u;
+    bit |= __shfl_xor(bit, 1);
+    bit |= __shfl_xor(bit, 2);
+    bit |= __shfl_xor(bit, 4);
+    if (!live) continue;
+    const uint32_t mask = bit;
+    const bool real = (mask >> k) & 1u;
+    const uint64_t local = q.off_l + 8ull * (q.g - q.sp.group_first) + k;
+    const uint64_t t = q.nbase + local;
+    const uint32_t parent = q.depth == (uint32_t)level + 1
+                                ? (uint32_t)q.off_p  // the root
+                                : (uint32_t)(q.off_p + 8ull * (q.gnext - q.spn_first) +
+                                             (uint32_t)(q.P & 7));
+    const K code = (kl.code(q.P) << 3) | (K)k;
+    uint32_t x, y, z;
+    demorton3<K>(code, x, y, z);
+    store_code(out, t, (uint64_t)code);
+    out.is_leaf[t] = (level == 1 || !real) ? 1 : 0;
+    // gridOrigin + width * Vector3f(x, y, z), per axis (SVO.cu:262)
+    out.origin[3 * t + 0] = q.ox + q.w * (float)x;
+    out.origin[3 * t + 1] = q.oy + q.w * (float)y;
+    out.origin[3 * t + 2] = q.oz + q.w * (float)z;
+    out.width[t] = q.w;
+    out.parent[t] = parent;
+    uint32_t cbase = 0;
+    const bool has_childs = real && level >= 2;
+    if (has_childs) {
+      const uint64_t i = (uint64_t)q.fc + __popc(mask & ((1u << k) - 1u));
+      cbase = (uint32_t)(q.off_c + 8ull * (i - q.sp.elem_first));
+    }
+    store_childs(out.childs + 8 * t, cbase, has_childs);
+  }
 }
 
 template <typename K>
@@ -682,7 +711,8 @@ void launch_emit_level_nodes(int level, KeyLayout<K> kl, const void* elem_l, uin
                              const MeshTreeState* state, const MeshNodeLayout* layout,
                              const MeshParams* params, NodeArrays out, hipStream_t s) {
   if (n_groups == 0) return;
-  hipLaunchKernelGGL(k_emit_nodes<K>, dim3(grid_for(8 * n_groups)), dim3(BLOCK), 0, s, level, kl,
+  hipLaunchKernelGGL(k_emit_nodes<K>, dim3(grid_for((8 * n_groups + NODES_PT - 1) / NODES_PT)),
+                     dim3(BLOCK), 0, s, level, kl,
                      (const K*)elem_l, n_elem_l, (const K*)elem_next, elem_group_next,
                      first_child_next, n_groups, spans, n_levels, state, layout, params, out);
 }

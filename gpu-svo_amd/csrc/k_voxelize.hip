@@ -270,18 +270,36 @@ __device__ inline bool cell_hit(const TriSetup& S, const RowTerms& R, int x) {
 // the float rounding of the test (2^-20 of the magnitudes involved, several
 // times the few half-ulp roundings of the float evaluation).  Tests whose
 // x-slope is tiny are not used.  Skipping cells outside the range therefore
-// never changes the set of hits.
-__device__ inline void row_range(const TriSetup& S, const RowTerms& R, int& xa, int& xb) {
-  double lo = S.g0[0], hi = S.g1[0];
+// never changes the set of hits.  The x-slopes and their reciprocals depend
+// on the triangle only (RangeConst, once per triangle); a root is the row
+// term times the reciprocal, within a few double ulps of the quotient, far
+// inside the margin.
+struct RangeConst {
+  double a, ia;        // plane test: slope n.x * u.x and its reciprocal
+  double b[6], ib[6];  // xy edges 0-2, zx edges 0-2
+};
+
+__device__ inline void range_const(const TriSetup& S, RangeConst& C) {
   const double ux = S.u[0];
+  C.a = (double)S.n[0] * ux;
+  C.ia = C.a != 0.0 ? 1.0 / C.a : 0.0;
+  for (int i = 0; i < 6; ++i) {
+    C.b[i] = (double)(i < 3 ? S.ne[0][i][0] : S.ne[2][i - 3][1]) * ux;
+    C.ib[i] = C.b[i] != 0.0 ? 1.0 / C.b[i] : 0.0;
+  }
+}
+
+__device__ inline void row_range(const TriSetup& S, const RangeConst& C, const RowTerms& R,
+                                 int& xa, int& xb) {
+  double lo = S.g0[0], hi = S.g1[0];
   const double xmax = (double)S.g1[0] + 1.0;
   {
-    const double a = (double)S.n[0] * ux;
+    const double a = C.a;
     const double d1 = S.d1, d2 = S.d2, A = R.A;
     const double mag = fabs(a) * xmax + fabs(A) + fabs(d1) + fabs(d2);
     if (fabs(a) > 1e-12 * (mag + 1e-30) && fabs(a) > 1e-15) {
-      const double err = mag * 0x1p-20 / fabs(a) + 2.0;
-      double x1 = (-fmax(d1, d2) - A) / a, x2 = (-fmin(d1, d2) - A) / a;
+      const double err = mag * 0x1p-20 * fabs(C.ia) + 2.0;
+      double x1 = (-fmax(d1, d2) - A) * C.ia, x2 = (-fmin(d1, d2) - A) * C.ia;
       if (x1 > x2) {
         const double t = x1;
         x1 = x2;
@@ -293,13 +311,13 @@ __device__ inline void row_range(const TriSetup& S, const RowTerms& R, int& xa, 
   }
   for (int i = 0; i < 6; ++i) {
     // xy: ne.x * px + rxy + de >= 0;  zx: rzx + ne.y * px + de >= 0
-    const double b = (double)(i < 3 ? S.ne[0][i][0] : S.ne[2][i - 3][1]) * ux;
+    const double b = C.b[i];
     const double c = i < 3 ? (double)R.rxy[i] + (double)S.de[0][i]
                            : (double)R.rzx[i - 3] + (double)S.de[2][i - 3];
     const double mag = fabs(b) * xmax + fabs(c) + fabs(i < 3 ? R.rxy[i] : R.rzx[i - 3]);
     if (!(fabs(b) > 1e-12 * (mag + 1e-30) && fabs(b) > 1e-30)) continue;
-    const double err = mag * 0x1p-20 / fabs(b) + 2.0;
-    const double root = -c / b;
+    const double err = mag * 0x1p-20 * fabs(C.ib[i]) + 2.0;
+    const double root = -c * C.ib[i];
     if (b > 0) lo = fmax(lo, floor(root - err));
     else hi = fmin(hi, ceil(root + err));
   }
@@ -537,6 +555,8 @@ __global__ __launch_bounds__(BLOCK) void k_vox_big(const MeshDev* meshes,
     load_tri(md, P, it.tri - md.tri_base, v);  // indices were checked when queued
     TriSetup S;
     tri_setup(v, P, S);
+    RangeConst RC;
+    range_const(S, RC);
     const uint64_t ny = S.g1[1] - S.g0[1] + 1, nz = S.g1[2] - S.g0[2] + 1;
     const K mesh_key = (K)m << cb;
     const uint64_t wbase = BITMAP ? out.bm_off[m] : 0;
@@ -547,7 +567,7 @@ __global__ __launch_bounds__(BLOCK) void k_vox_big(const MeshDev* meshes,
       if (r < ny * nz) {
         y = S.g0[1] + (int)(r % ny);
         z = S.g0[2] + (int)(r / ny);
-        if (row_terms(S, y, z, R)) row_range(S, R, xa, xb);
+        if (row_terms(S, y, z, R)) row_range(S, RC, R, xa, xb);
       }
       for (int x = xa; __any(x <= xb); ++x) {
         const bool hit = (x <= xb) & cell_hit(S, R, x);
