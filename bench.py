@@ -156,6 +156,10 @@ def run_batch_workload(args):
         ctx.set_option("batch_lanes", args.batch_lanes)
     dms = [ctx.upload_mesh(P, T) for P, T, _ in meshes]
     descs = [dm.desc(res) for dm, (_, _, res) in zip(dms, meshes)]
+    tuning = None
+    if args.autotune:
+        log("bench: tuning search on the batch (untimed)")
+        tuning = ctx.autotune(descs, on_device=True, reps=5)
 
     sizes = {}
 
@@ -231,6 +235,7 @@ def run_batch_workload(args):
         "batched_vs_plain": round(ms_plain / ms_batch, 3),
         "sizes": dict({"voxels": nv, "nodes": nn, "hits": stats["n_hits"]}, **sizes),
         "passes": ctx.get_option("batch_lanes"),
+        "tuning": tuning,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "dedup_gbs": dedup_gbs(stats, stages),
         "dedup_path": "bitmap" if stats["bitmap_bytes"] else "sort",
@@ -430,6 +435,8 @@ def main():
     ap.add_argument("--max-batch", type=int, default=32)
     ap.add_argument("--max-wait-us", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batch workload: run the library's tuning search on the batch first")
     ap.add_argument("--batch-lanes", type=int, default=0,
                     help="concurrent passes a batch is split into (0: library default)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,

@@ -1,5 +1,7 @@
 // api.cpp -- the C ABI declared in include/svo/svo.h.
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -283,13 +285,133 @@ int svo_build_batch(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params*
   std::vector<svo_tree*> tr;
   std::vector<int> st;
   const int s = run_batch(ctx->impl, in, tr, st);
+  // a failed pass of a split batch marks its own meshes; the others' trees
+  // are returned
   int agg = s;
   for (uint32_t i = 0; i < n; ++i) {
-    trees[i] = (s == SVO_OK) ? tr[i] : nullptr;
-    if (statuses) statuses[i] = (s == SVO_OK) ? st[i] : s;
+    trees[i] = tr[i];
+    if (statuses) statuses[i] = st[i];
     if (agg == SVO_OK && st[i] != SVO_OK) agg = st[i];
   }
   return agg;
+}
+
+// ---- tuning search and persistence
+namespace {
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+}  // namespace
+
+int svo_ctx_autotune(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params* params,
+                     uint32_t n, int meshes_on_device, uint32_t reps, svo_tune_result* out) {
+  if (!ctx || !n || !meshes || !params) return SVO_E_INVALID;
+  if (reps == 0) reps = 3;
+  Context& C = ctx->impl;
+  Tuning base;
+  {
+    std::lock_guard<std::mutex> lk(C.mu);
+    base = C.tuning;
+  }
+  std::vector<svo_tree*> trees(n, nullptr);
+  std::vector<int> st(n, SVO_OK);
+  // mean wall time of one build of the sample under tuning t
+  auto trial = [&](const Tuning& t, double& ms) -> int {
+    {
+      std::lock_guard<std::mutex> lk(C.mu);
+      C.tuning = t;
+    }
+    std::vector<double> times;
+    for (uint32_t r = 0; r <= reps; ++r) {  // the first build warms up
+      const double t0 = now_ms();
+      const int rc = svo_build_batch(ctx, meshes, params, n, meshes_on_device, trees.data(),
+                                     st.data());
+      const double t1 = now_ms();
+      for (auto& tr : trees) {
+        svo_tree_free(tr);
+        tr = nullptr;
+      }
+      if (rc != SVO_OK) return rc;
+      if (r > 0) times.push_back(t1 - t0);
+    }
+    std::sort(times.begin(), times.end());
+    ms = times[times.size() / 2];  // median: one slow build does not decide
+    return SVO_OK;
+  };
+  double best_ms = 0, default_ms = 0;
+  int rc = trial(base, default_ms);
+  if (rc != SVO_OK) {
+    std::lock_guard<std::mutex> lk(C.mu);
+    C.tuning = base;
+    return rc;
+  }
+  Tuning best = base;
+  best_ms = default_ms;
+  uint32_t trials = 1;
+  const uint32_t smalls[] = {64, 128, 256, 512, 1024};
+  const uint32_t lanes[] = {1, 2, 3, 4};
+  for (uint32_t sm : smalls)
+    for (uint32_t ln : lanes) {
+      if (ln > 1 && n < 2 * base.batch_min_meshes * (ln > 2 ? 2 : 1)) continue;
+      Tuning t = base;
+      t.small_max_cells = sm;
+      t.batch_lanes = ln;
+      if (t.small_max_cells == base.small_max_cells && t.batch_lanes == base.batch_lanes) continue;
+      double ms = 0;
+      rc = trial(t, ms);
+      ++trials;
+      if (rc != SVO_OK) break;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = t;
+      }
+    }
+  {
+    std::lock_guard<std::mutex> lk(C.mu);
+    C.tuning = rc == SVO_OK ? best : base;
+  }
+  if (out) {
+    out->small_max_cells = best.small_max_cells;
+    out->batch_lanes = best.batch_lanes;
+    out->best_ms = best_ms;
+    out->default_ms = default_ms;
+    out->n_trials = trials;
+  }
+  return rc;
+}
+
+int svo_ctx_save_tuning(svo_ctx* ctx, const char* path) {
+  if (!ctx || !path) return SVO_E_INVALID;
+  FILE* f = std::fopen(path, "w");
+  if (!f) return SVO_E_IO;
+  for (const OptionRef& o : kOptions) {
+    int64_t v = 0;
+    if (svo_ctx_get_option(ctx, o.name, &v) == SVO_OK)
+      std::fprintf(f, "%s %lld\n", o.name, (long long)v);
+  }
+  return std::fclose(f) == 0 ? SVO_OK : SVO_E_IO;
+}
+
+int svo_ctx_load_tuning(svo_ctx* ctx, const char* path) {
+  if (!ctx || !path) return SVO_E_INVALID;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return SVO_E_IO;
+  char name[64];
+  long long v = 0;
+  int rc = SVO_OK;
+  while (rc == SVO_OK) {
+    const int got = std::fscanf(f, "%63s %lld", name, &v);
+    if (got == EOF) break;
+    if (got != 2) {
+      rc = SVO_E_PARSE;
+      break;
+    }
+    rc = svo_ctx_set_option(ctx, name, (int64_t)v);
+  }
+  std::fclose(f);
+  return rc;
 }
 
 int svo_tree_info_get(const svo_tree* tree, svo_tree_info* out) {

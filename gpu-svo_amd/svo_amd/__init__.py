@@ -78,6 +78,11 @@ class Completion(C.Structure):
                 ("t_done_s", C.c_double)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("small_max_cells", C.c_uint32), ("batch_lanes", C.c_uint32),
+                ("best_ms", C.c_double), ("default_ms", C.c_double), ("n_trials", C.c_uint32)]
+
+
 # Every exported entry point and its ctypes signature (kept in step with
 # include/svo/svo.h; tests check the two agree).
 _SIGS = {
@@ -85,6 +90,10 @@ _SIGS = {
     "svo_ctx_destroy": (C.c_int, [C.c_void_p]),
     "svo_ctx_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "svo_ctx_set_option": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "svo_ctx_autotune": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_int,
+                                   C.c_uint32, C.POINTER(TuneResult)]),
+    "svo_ctx_save_tuning": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "svo_ctx_load_tuning": (C.c_int, [C.c_void_p, C.c_char_p]),
     "svo_ctx_get_option": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64)]),
     "svo_ctx_last_stage_ms": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_char_p), C.c_int,
@@ -390,10 +399,8 @@ class Context:
                "build_device")
         return Tree(h.value, self)
 
-    def build_batch(self, meshes, semantics=SEM_REFERENCE, key_bits=0, flags=0,
-                    on_device=False):
-        """meshes: list of (points, tris, res) (host arrays) or, with on_device,
-        (points_ptr, n_points, tris_ptr, n_tris, res).  Returns (trees, statuses)."""
+    @staticmethod
+    def _batch_args(meshes, semantics, key_bits, flags, on_device):
         n = len(meshes)
         descs = (MeshDesc * n)()
         params = (Params * n)()
@@ -408,11 +415,36 @@ class Context:
                 keep.append(k)
                 descs[i] = d
             params[i] = make_params(res, semantics, key_bits, flags)
+        return descs, params, keep
+
+    def build_batch(self, meshes, semantics=SEM_REFERENCE, key_bits=0, flags=0,
+                    on_device=False):
+        """meshes: list of (points, tris, res) (host arrays) or, with on_device,
+        (points_ptr, n_points, tris_ptr, n_tris, res).  Returns (trees, statuses)."""
+        n = len(meshes)
+        descs, params, keep = self._batch_args(meshes, semantics, key_bits, flags, on_device)
         out = (C.c_void_p * n)()
         st = (C.c_int * n)()
         lib().svo_build_batch(self._h, descs, params, n, 1 if on_device else 0, out, st)
         trees = [Tree(out[i], self) if st[i] == OK and out[i] else None for i in range(n)]
         return trees, [st[i] for i in range(n)]
+
+    def autotune(self, meshes, semantics=SEM_REFERENCE, key_bits=0, flags=0, on_device=False,
+                 reps=3) -> dict:
+        """svo_ctx_autotune on a sample batch (same forms as build_batch); the
+        fastest options stay set on the context."""
+        n = len(meshes)
+        descs, params, keep = self._batch_args(meshes, semantics, key_bits, flags, on_device)
+        r = TuneResult()
+        _check(lib().svo_ctx_autotune(self._h, descs, params, n, 1 if on_device else 0, reps,
+                                      C.byref(r)), "autotune")
+        return {k: getattr(r, k) for k, _ in TuneResult._fields_}
+
+    def save_tuning(self, path: str):
+        _check(lib().svo_ctx_save_tuning(self._h, path.encode()), "save tuning")
+
+    def load_tuning(self, path: str):
+        _check(lib().svo_ctx_load_tuning(self._h, path.encode()), "load tuning")
 
 
 class DeviceMesh:

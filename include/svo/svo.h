@@ -166,6 +166,21 @@ int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap,
  * Unknown names or out-of-range values give SVO_E_INVALID. */
 int svo_ctx_set_option(svo_ctx* ctx, const char* name, int64_t value);
 int svo_ctx_get_option(svo_ctx* ctx, const char* name, int64_t* value);
+/* Tuning search (no counterpart in the reference): builds the sample batch
+ * `reps` times (after one warm-up; the median time counts) under the current
+ * options and under each combination of small_max_cells in {64, 128, 256, 512, 1024} and
+ * batch_lanes in {1, 2, 3, 4}, keeps the fastest on the context and reports
+ * it.  The trees built are freed. */
+typedef struct {
+  uint32_t small_max_cells, batch_lanes; /* chosen */
+  double best_ms, default_ms;            /* median wall ms per build of the sample */
+  uint32_t n_trials;
+} svo_tune_result;
+int svo_ctx_autotune(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params* params,
+                     uint32_t n, int meshes_on_device, uint32_t reps, svo_tune_result* out);
+/* Options as "name value" lines (every option svo_ctx_set_option knows). */
+int svo_ctx_save_tuning(svo_ctx* ctx, const char* path);
+int svo_ctx_load_tuning(svo_ctx* ctx, const char* path);
 /* Sizes of the last build call on this context, summed over its passes. */
 typedef struct {
   uint64_t n_meshes, n_tris, box_cells, n_hits, n_voxels, level_elems, n_nodes;

@@ -96,3 +96,36 @@ def test_batch_single_voxel_meshes(gpu_ctx):
     for m, t in zip(meshes, trees):
         compare(t, po.build(*m))
         t.free()
+
+
+def test_autotune_keeps_results_and_round_trips(gpu_ctx, tmp_path):
+    """The tuning search picks options on a sample batch; trees built under the
+    chosen options equal the default context's; options survive a save/load."""
+    import svo_amd
+    meshes = _meshes(36, seed=23)
+    print(f"autotune on {len(meshes)} meshes", flush=True)
+    ctx = svo_amd.Context(0)
+    try:
+        r = ctx.autotune(meshes, reps=1)
+        assert r["n_trials"] > 1 and r["best_ms"] <= r["default_ms"] + 1e-9
+        assert ctx.get_option("small_max_cells") == r["small_max_cells"]
+        assert ctx.get_option("batch_lanes") == r["batch_lanes"]
+        path = str(tmp_path / "tuning.txt")
+        ctx.save_tuning(path)
+        ctx2 = svo_amd.Context(0)
+        try:
+            ctx2.load_tuning(path)
+            for name in ("small_max_cells", "batch_lanes", "dedup_path", "bitmap_max_bytes"):
+                assert ctx2.get_option(name) == ctx.get_option(name)
+        finally:
+            ctx2.close()
+        trees, st = ctx.build_batch(meshes)
+        ref, st2 = gpu_ctx.build_batch(meshes)
+        assert st == st2 and all(s == 0 for s in st)
+        for i, (a, b) in enumerate(zip(trees, ref)):
+            for f in ALL:
+                assert_same(f"mesh {i} {f}", a.field(f), b.field(f))
+            a.free()
+            b.free()
+    finally:
+        ctx.close()

@@ -42,10 +42,11 @@ def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "sizes.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "svo/svo.h"\n'
-        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\","
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n\","
         "sizeof(svo_mesh_desc), sizeof(svo_params), sizeof(svo_tree_info),"
         "sizeof(svo_server_cfg), sizeof(svo_completion), sizeof(svo_vertex), sizeof(svo_edge),"
-        "sizeof(svo_node_rec), offsetof(svo_tree_info, level_nodes), offsetof(svo_node_rec, neighbors));"
+        "sizeof(svo_node_rec), offsetof(svo_tree_info, level_nodes), offsetof(svo_node_rec, neighbors),"
+        "sizeof(svo_tune_result), sizeof(svo_pass_stats));"
         "return 0;}\n")
     exe = tmp_path / "sizes"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
@@ -55,7 +56,8 @@ def test_struct_layouts_match_c(tmp_path):
             C.sizeof(svo_amd.ServerCfg), C.sizeof(svo_amd.Completion),
             svo_amd.VERT_DTYPE.itemsize, svo_amd.EDGE_DTYPE.itemsize,
             svo_amd.NODE_DTYPE.itemsize, svo_amd.TreeInfo.level_nodes.offset,
-            svo_amd.NODE_DTYPE.fields["neighbors"][1]]
+            svo_amd.NODE_DTYPE.fields["neighbors"][1], C.sizeof(svo_amd.TuneResult),
+            11 * 8 + 4 * 4 + 2 * 8]
     assert got == want
     assert got[5:8] == [16, 28, 168]
 
