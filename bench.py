@@ -363,10 +363,10 @@ def run_served_workload(args):
     # warm the lanes (kernel code objects, workspaces)
     P0, T0 = meshgen.icosphere(3)
     ctx.build(P0, T0, 64).free()
-    srv = svo_amd.Server(ctx, mem_budget=64 << 30, max_batch=args.max_batch, n_lanes=2,
-                         max_wait_us=args.max_wait_us)
+    srv = svo_amd.Server(ctx, mem_budget=64 << 30, max_batch=args.max_batch,
+                         n_lanes=args.lanes, max_wait_us=args.max_wait_us)
     rng = np.random.default_rng(5)
-    lat, bsz = [], []
+    lat, bsz, wait = [], [], []
     done = 0
     queries = 0
     t_start = time.perf_counter()
@@ -381,6 +381,7 @@ def run_served_workload(args):
             if d.status != 0:
                 raise RuntimeError(f"request {d.request_id} failed: {d.status}")
             lat.append(d.t_done - d.t_submit)
+            wait.append(d.t_start - d.t_submit)
             bsz.append(d.batch_size)
             nn = d.tree.info.n_nodes
             q = d.tree.query_neighbors(rng.integers(0, nn, 1000).astype(np.uint32),
@@ -402,6 +403,10 @@ def run_served_workload(args):
                                f"per resident tree", "parallelism": "single GPU"},
         "p50_ms": round(float(np.percentile(lat, 50)), 3),
         "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "queue_wait_p50_ms": round(float(np.percentile(np.array(wait) * 1e3, 50)), 3),
+        "queue_wait_p99_ms": round(float(np.percentile(np.array(wait) * 1e3, 99)), 3),
+        "server": {"lanes": args.lanes, "max_batch": args.max_batch,
+                   "max_wait_us": args.max_wait_us},
         "mean_batch": round(float(np.mean(bsz)), 2), "queries": queries,
         "roofline": None, "cpu_baseline": None,
     }
@@ -434,6 +439,7 @@ def main():
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--max-batch", type=int, default=32)
     ap.add_argument("--max-wait-us", type=int, default=1000)
+    ap.add_argument("--lanes", type=int, default=2, help="served: pipeline lanes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--autotune", action="store_true",
                     help="batch workload: run the library's tuning search on the batch first")
