@@ -487,6 +487,25 @@ int svo_tree_copy_field(const svo_tree* tree, int field, void* dst, uint64_t dst
   return SVO_OK;
 }
 
+int svo_tree_copy_voxels(const svo_tree* tree, void* dst, uint64_t dst_bytes) {
+  return svo_tree_copy_field(tree, SVO_FIELD_VOXELS, dst, dst_bytes);
+}
+int svo_tree_copy_nodes(const svo_tree* tree, svo_node_rec* dst, uint64_t dst_bytes) {
+  return svo_tree_copy_field(tree, SVO_FIELD_NODES_AOS, dst, dst_bytes);
+}
+int svo_tree_copy_neighbors(const svo_tree* tree, uint32_t* dst, uint64_t dst_bytes) {
+  return svo_tree_copy_field(tree, SVO_FIELD_NEIGHBORS, dst, dst_bytes);
+}
+int svo_tree_copy_vertices(const svo_tree* tree, svo_vertex* dst, uint64_t dst_bytes) {
+  return svo_tree_copy_field(tree, SVO_FIELD_VERTICES, dst, dst_bytes);
+}
+int svo_tree_copy_edges(const svo_tree* tree, svo_edge* dst, uint64_t dst_bytes) {
+  return svo_tree_copy_field(tree, SVO_FIELD_EDGES, dst, dst_bytes);
+}
+int svo_tree_copy_faces(const svo_tree* tree, svo_edge* dst, uint64_t dst_bytes) {
+  return svo_tree_copy_field(tree, SVO_FIELD_FACES, dst, dst_bytes);
+}
+
 int svo_write_obj(const svo_tree* tree, const char* dir, const char* stem, char* path_out,
                   size_t path_cap) {
   if (!tree || !stem) return SVO_E_INVALID;

@@ -117,6 +117,12 @@ _SIGS = {
     "svo_tree_field_bytes": (C.c_uint64, [C.c_void_p, C.c_int]),
     "svo_tree_copy_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_uint64]),
     "svo_tree_device_ptr": (C.c_void_p, [C.c_void_p, C.c_int]),
+    "svo_tree_copy_voxels": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "svo_tree_copy_nodes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "svo_tree_copy_neighbors": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "svo_tree_copy_vertices": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "svo_tree_copy_edges": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
+    "svo_tree_copy_faces": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "svo_write_obj": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_size_t]),
     "svo_write_voxels_obj": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p,
                                        C.c_size_t]),

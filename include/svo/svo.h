@@ -225,6 +225,20 @@ int svo_tree_info_get(const svo_tree* tree, svo_tree_info* out);
 uint64_t svo_tree_field_bytes(const svo_tree* tree, int field);
 /* Copies a field to host memory. */
 int svo_tree_copy_field(const svo_tree* tree, int field, void* dst, uint64_t dst_bytes);
+/* Named read-outs of the reference's arrays (each svo_tree_copy_field of one
+ * field; dst_bytes at least svo_tree_field_bytes of it):
+ *   voxels     sorted unique Morton codes (compactArray output, SVO.cu:405)
+ *   nodes      svo_node_rec[n_nodes] (svoNodeArray, SVO.h:47)
+ *   neighbors  uint32[n_nodes][27] (SVONode::neighbors, SVO.h:33)
+ *   vertices   svo_vertex[n_vertices] (nodeVertexArray, SVO.h:50)
+ *   edges      svo_edge[n_edges] (nodeEdgeArray, SVO.h:51)
+ *   faces      svo_edge[n_faces] (intended semantics only) */
+int svo_tree_copy_voxels(const svo_tree* tree, void* dst, uint64_t dst_bytes);
+int svo_tree_copy_nodes(const svo_tree* tree, svo_node_rec* dst, uint64_t dst_bytes);
+int svo_tree_copy_neighbors(const svo_tree* tree, uint32_t* dst, uint64_t dst_bytes);
+int svo_tree_copy_vertices(const svo_tree* tree, svo_vertex* dst, uint64_t dst_bytes);
+int svo_tree_copy_edges(const svo_tree* tree, svo_edge* dst, uint64_t dst_bytes);
+int svo_tree_copy_faces(const svo_tree* tree, svo_edge* dst, uint64_t dst_bytes);
 /* Device pointer of a field, valid until svo_tree_free (NULL for AOS/LEVEL). */
 const void* svo_tree_device_ptr(const svo_tree* tree, int field);
 /* writeTree, SVO.cu:737-753 + IO.h:23-90: writes <dir>/<stem>_<depth>_tree.obj

@@ -170,3 +170,23 @@ def test_options_rejected(gpu_ctx):
         with pytest.raises(svo_amd.SvoError) as e:
             gpu_ctx.set_option(name, value)
         assert e.value.status == svo_amd.E_INVALID
+
+
+def test_named_readouts(gpu_ctx, bunny):
+    """svo_tree_copy_{voxels,nodes,neighbors,vertices,edges,faces} return the
+    same bytes as the corresponding field copies."""
+    import svo_amd
+    P, T = bunny
+    tree = gpu_ctx.build(P, T, 32, semantics=1)
+    L = svo_amd.lib()
+    try:
+        for name, field in (("voxels", "voxels"), ("nodes", "nodes_aos"),
+                            ("neighbors", "neighbors"), ("vertices", "vertices"),
+                            ("edges", "edges"), ("faces", "faces")):
+            ref = tree.field(field)
+            buf = np.zeros_like(ref)
+            st = getattr(L, f"svo_tree_copy_{name}")(tree._h, buf.ctypes.data, buf.nbytes)
+            assert st == 0, name
+            assert buf.tobytes() == ref.tobytes(), name
+    finally:
+        tree.free()
