@@ -412,6 +412,74 @@ def run_served_workload(args):
     }
 
 
+def run_io_workload(args):
+    """OBJ in and out at speed: parse a large mesh file with the library's
+    parallel reader (BaseModel::loadOBJ rules) and write a tree with its
+    parallel byte-exact writer, each beside the oracle's sequential one."""
+    import tempfile
+    import numpy as np
+    import pyoracle as po
+    import svo_amd
+    from svo_amd import meshgen
+    d = tempfile.mkdtemp(prefix="svo_io_")
+    P, T = meshgen.noisy_icosphere(7, seed=1)
+    src = os.path.join(d, "ico7.obj")
+    with open(src, "w") as f:
+        f.write("".join("v %.9g %.9g %.9g\n" % tuple(p) for p in P.astype(np.float64)))
+        f.write("".join("f %d %d %d\n" % tuple(t + 1) for t in T))
+    in_bytes = os.path.getsize(src)
+    log(f"bench: parsing {in_bytes / 1e6:.1f} MB")
+    def best_of(fn, k):
+        ts = []
+        for _ in range(k):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+    t_parse = best_of(lambda: svo_amd.load_obj(src), max(1, args.steps))
+    t_parse_ref = best_of(lambda: po.load_obj(src), 1)
+    P2, T2 = svo_amd.load_obj(src)
+    assert np.array_equal(P2, po.load_obj(src)[1]) and np.array_equal(T2, po.load_obj(src)[2])
+    ctx = svo_amd.Context(0)
+    tree = ctx.build(P, T, 256)
+    out = os.path.join(d, "gpu")
+    os.makedirs(out)
+    t_write = best_of(lambda: tree.write_obj(out, "ico7"), max(1, args.steps))
+    path = tree.write_obj(out, "ico7")
+    out_bytes = os.path.getsize(path)
+    ref_path = os.path.join(d, "ref_tree.obj")
+    t0 = time.perf_counter()
+    st, _ = po.write_obj(P, T, 256, ref_path)
+    t_write_ref = time.perf_counter() - t0  # includes the oracle's build
+    same = open(path, "rb").read() == open(ref_path, "rb").read()
+    n_nodes = tree.info.n_nodes
+    tree.free()
+    ctx.close()
+    for f in os.listdir(out):
+        os.remove(os.path.join(out, f))
+    os.rmdir(out)
+    for f in (src, ref_path):
+        os.remove(f)
+    os.rmdir(d)
+    return {
+        "metric": "OBJ parse and tree write MB/s", "value": round(out_bytes / t_write / 1e6, 1),
+        "unit": "MB/s written", "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "ms_per_step": round(t_write * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "text", "data": "synthetic",
+        "config": {"workload": "parse icosphere-7 + noise as OBJ text (327,680 faces); write its "
+                               "res-256 tree as <stem>_<depth>_tree.obj",
+                   "parallelism": "host threads"},
+        "parse_mb_s": round(in_bytes / t_parse / 1e6, 1),
+        "parse_mb_s_oracle": round(in_bytes / t_parse_ref / 1e6, 1),
+        "write_mb_s": round(out_bytes / t_write / 1e6, 1),
+        "write_s_oracle_incl_build": round(t_write_ref, 2),
+        "write_mb_s_oracle_incl_build": round(out_bytes / t_write_ref / 1e6, 1),
+        "sizes": {"obj_in_mb": round(in_bytes / 1e6, 1), "tree_obj_mb": round(out_bytes / 1e6, 1),
+                  "nodes": n_nodes},
+        "byte_identical_to_oracle": same, "roofline": None, "cpu_baseline": None,
+    }
+
+
 def copy_probe(nbytes):
     """One device-to-device copy of nbytes (reads nbytes, writes nbytes)."""
     import ctypes as C
@@ -435,7 +503,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--workload", default="batch",
-                    choices=["batch", "single", "soup", "soup2048", "served"])
+                    choices=["batch", "single", "soup", "soup2048", "served", "io"])
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--max-batch", type=int, default=32)
     ap.add_argument("--max-wait-us", type=int, default=1000)
@@ -476,6 +544,8 @@ def main():
             out = run_single_workload(args, "soup2048", P, T, 2048, 64, flags=1,
                                       desc="2,000,000-triangle soup, res 2048, uint64 keys, "
                                            "nodes and neighbors without vertex/edge records")
+    elif args.workload == "io":
+        out = run_io_workload(args)
     else:
         out = run_served_workload(args)
     print(json.dumps(out), flush=True)

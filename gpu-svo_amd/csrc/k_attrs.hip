@@ -143,22 +143,29 @@ __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
     a[u].fc = first_child_next[a[u].g];
     a[u].gnext = group_next[a[u].g];
   }
+  auto mesh_fields = [&](NodeIn<K>& q, uint32_t m) {
+    q.depth = state[m].depth;
+    const MeshNodeLayout* L = layout + m;
+    q.nn = L->n_nodes;
+    q.nbase = L->node_base;
+    q.off_l = L->level_off[level - 1];
+    q.off_p = L->level_off[level];
+    q.off_c = level >= 2 ? L->level_off[level - 2] : 0;
+    q.w = L->width[level - 1];
+    q.sp = spans[(uint64_t)m * n_levels + (level - 1)];
+    q.spn_first = spans[(uint64_t)m * n_levels + level].group_first;
+    q.ox = params[m].origin[0];
+    q.oy = params[m].origin[1];
+    q.oz = params[m].origin[2];
+  };
 #pragma unroll
   for (int u = 0; u < NODES_PT; ++u) {  // round 2
     const uint32_t m = kl.mesh(a[u].P);
-    a[u].depth = state[m].depth;
-    const MeshNodeLayout* L = layout + m;
-    a[u].nn = L->n_nodes;
-    a[u].nbase = L->node_base;
-    a[u].off_l = L->level_off[level - 1];
-    a[u].off_p = L->level_off[level];
-    a[u].off_c = level >= 2 ? L->level_off[level - 2] : 0;
-    a[u].w = L->width[level - 1];
-    a[u].sp = spans[(uint64_t)m * n_levels + (level - 1)];
-    a[u].spn_first = spans[(uint64_t)m * n_levels + level].group_first;
-    a[u].ox = params[m].origin[0];
-    a[u].oy = params[m].origin[1];
-    a[u].oz = params[m].origin[2];
+    // a wave's nodes nearly always share one mesh: its per-mesh records are
+    // then read once for the wave (scalar loads) instead of once per lane
+    const uint32_t m0 = __builtin_amdgcn_readfirstlane(m);
+    if (__all(m == m0)) mesh_fields(a[u], m0);
+    else mesh_fields(a[u], m);
     const uint64_t j = (uint64_t)a[u].fc + k;
     a[u].e = j < n_elem_l ? elem_l[j] : (K)0;
   }
