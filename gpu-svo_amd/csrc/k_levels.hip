@@ -220,15 +220,8 @@ __global__ __launch_bounds__(BLOCK) void k_first_pass(
 // are the parent groups (the eight children of parent p are codes 8p..8p+7,
 // one byte).  count -> scan of tile counts -> emit, each over tiles of 2048
 // words.
-constexpr int WAVES_LV = BLOCK / 64;
 constexpr int BM_WPT = 8;                 // words per thread (two 16-byte loads)
 
-// Orders LDS accesses of different lanes of one wave.
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 constexpr int BM_TILE = BLOCK * BM_WPT;   // 2048 words
 
 __device__ inline uint32_t nonzero_bytes(uint32_t w) {
@@ -335,27 +328,17 @@ __device__ inline uint32_t mesh_of_word(const uint64_t* bm_off, uint32_t n_meshe
   return lo;
 }
 
-// Position of the k-th (0-based) set bit of w; w has more than k set bits.
-__device__ inline uint32_t select_bit(uint32_t w, uint32_t k) {
-  uint32_t pos = 0;
-#pragma unroll
-  for (int half = 16; half >= 1; half >>= 1) {
-    const uint32_t lo = __popc(w & ((1u << half) - 1u));
-    if (k >= lo) {
-      k -= lo;
-      w >>= half;
-      pos += half;
-    }
-  }
-  return pos;
-}
+// One tile of 2048 words.  The tile's nonzero words are compacted into LDS
+// in word order with the exclusive prefixes of their set bits (low 17 bits)
+// and nonzero bytes (high 15 bits).  Then, 256 nonzero words per round, a
+// lane takes one word and walks its set bits (unique keys, in order) and its
+// nonzero bytes (groups) into LDS buffers, which the block writes out
+// contiguously.  A lane per nonzero word keeps most lanes busy (a lane per
+// bit would leave about four in five idle at surface densities), and staging
+// turns the lanes' strided stores into whole cache lines.  A round with more
+// than EMIT_CAP keys (dense tiles) stores its keys directly.
+constexpr uint32_t EMIT_CAP = 2048;
 
-
-// One tile of 2048 words.  The tile's words and the exclusive prefixes of
-// their set bits and nonzero bytes go to LDS; then the outputs are produced
-// in output order so that the stores are contiguous: unique key o is the
-// k-th set bit of the word whose bit prefix holds o, group g the k-th nonzero
-// byte of the word whose byte prefix holds g.
 template <typename K>
 __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_t n_words,
                                                    const uint64_t* bm_off, uint32_t n_meshes,
@@ -363,8 +346,10 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_
                                                    const uint32_t* tile_g, K* elem1, K* elem2,
                                                    uint32_t* first_child2) {
   __shared__ uint64_t tmp[4];
-  __shared__ uint32_t s_word[BM_TILE], s_upre[BM_TILE], s_gpre[BM_TILE];
-  __shared__ int32_t s_own[WAVES_LV][64];
+  __shared__ uint32_t s_word[BM_TILE], s_pre[BM_TILE];
+  __shared__ uint16_t s_idx[BM_TILE];
+  __shared__ K s_key[EMIT_CAP], s_par[BLOCK * 4];
+  __shared__ uint32_t s_fc[BLOCK * 4];
   __shared__ uint32_t s_mesh;
   __shared__ uint64_t s_moff[2];  // word range [s_moff[0], s_moff[1]) of mesh s_mesh
   const uint64_t tw0 = (uint64_t)blockIdx.x * BM_TILE;
@@ -377,101 +362,117 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_
   }
   uint32_t w[BM_WPT];
   load_words(bits, n_words, tw0 + l0, w);
-  uint32_t u = 0, g = 0;
+  // packed counts: set bits in 0-23, nonzero bytes in 24-43, nonzero words in 44-63
+  uint64_t cnt = 0;
 #pragma unroll
-  for (int j = 0; j < BM_WPT; ++j) {
-    u += __popc(w[j]);
-    g += nonzero_bytes(w[j]);
-  }
+  for (int j = 0; j < BM_WPT; ++j)
+    cnt += (uint64_t)__popc(w[j]) | ((uint64_t)nonzero_bytes(w[j]) << 24) |
+           ((uint64_t)(w[j] != 0) << 44);
   uint64_t tot;
-  const uint64_t ex = block_excl_scan256(((uint64_t)g << 32) | u, tmp, tot);
+  const uint64_t ex = block_excl_scan256(cnt, tmp, tot);
   {
-    uint32_t pu = (uint32_t)ex, pg = (uint32_t)(ex >> 32);
+    uint32_t pu = (uint32_t)(ex & 0xFFFFFFu), pg = (uint32_t)((ex >> 24) & 0xFFFFFu),
+             pz = (uint32_t)(ex >> 44);
 #pragma unroll
     for (int j = 0; j < BM_WPT; ++j) {
-      s_word[l0 + j] = w[j];
-      s_upre[l0 + j] = pu;
-      s_gpre[l0 + j] = pg;
+      if (w[j]) {
+        s_word[pz] = w[j];
+        s_pre[pz] = pu | (pg << 17);  // pu <= 65535, pg <= 8191 for a nonzero word
+        s_idx[pz] = (uint16_t)(l0 + j);
+        ++pz;
+      }
       pu += __popc(w[j]);
       pg += nonzero_bytes(w[j]);
     }
   }
   __syncthreads();
+  const uint32_t nz = (uint32_t)(tot >> 44);
+  const uint32_t pre_end = (uint32_t)(tot & 0xFFFFFFu) | ((uint32_t)((tot >> 24) & 0xFFFFFu) << 17);
   const uint32_t bu = tile_u[blockIdx.x], bg = tile_g[blockIdx.x];
   const uint32_t m0 = s_mesh;
   const uint64_t moff0 = s_moff[0], moff1 = s_moff[1];
-  // mesh of word gw and its first word: the tile's first mesh unless gw lies
-  // past it (a tile spanning several small meshes)
-  auto mesh_at = [&](uint64_t gw, uint32_t& m, uint64_t& off) {
-    if (gw < moff1) {
-      m = m0;
-      off = moff0;
-      return;
-    }
-    m = m0 + 1;
-    while (m + 1 < n_meshes && gw >= bm_off[m + 1]) ++m;
-    off = bm_off[m];
-  };
-  // Each wave emits the outputs of its quarter of the tile, 64 words at a
-  // time.  Lane j holds word i0 + j and the tile prefixes of its set bits
-  // (pu) and nonzero bytes (pg).  The window's outputs go out 64 per round:
-  // a lane whose range starts inside the round marks that slot with its lane
-  // id, a max-scan over the slots gives every output slot its word, and the
-  // output is the k-th set bit (or nonzero byte) of that word.  Consecutive
-  // lanes store consecutive outputs.
-  const uint32_t wv = threadIdx.x >> 6, lane = __lane_id();
-  int32_t* own = s_own[wv];
-  for (uint32_t i0 = wv * (BM_TILE / WAVES_LV); i0 < (wv + 1) * (BM_TILE / WAVES_LV); i0 += 64) {
-    const uint32_t idx = i0 + lane;
-    const uint32_t word = s_word[idx];
-    const uint32_t pu = s_upre[idx], pg = s_gpre[idx];
-    const uint32_t cu = __popc(word), cg = nonzero_bytes(word);
-    uint32_t m;
-    uint64_t off;
-    mesh_at(tw0 + idx, m, off);
-    const K hi = ((K)m << kl.cb) | ((K)(tw0 + idx - off) << 5);
-#pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
-      // pass 0: unique keys (set bits); pass 1: groups (nonzero bytes)
-      const uint32_t p = pass == 0 ? pu : pg, c = pass == 0 ? cu : cg;
-      const uint32_t start = __shfl(p, 0), end = __shfl(p + c, 63);
-      for (uint32_t base = start; base < end; base += 64) {
-        own[lane] = 0;
-        wave_sync();
-        if (c > 0) {
-          const uint32_t first = p > base ? p : base;
-          if (first < p + c && first < base + 64) own[first - base] = (int32_t)lane;
-        }
-        wave_sync();
-        int32_t j = own[lane];
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-          const int32_t t = __shfl_up(j, d);
-          if (lane >= (uint32_t)d && t > j) j = t;
-        }
-        wave_sync();
-        const uint32_t o = base + lane;
-        const uint32_t wj = __shfl(word, j), pj = __shfl(p, j);
-        const K hj = __shfl(hi, j);
-        if (o < end) {
-          if (pass == 0) {
-            elem1[(uint64_t)bu + o] = hj | (K)select_bit(wj, o - pj);
-          } else {
-            uint32_t k = o - pj, y = 0;
-            for (; y < 4; ++y) {
-              if ((wj >> (8 * y)) & 0xFFu) {
-                if (k == 0) break;
-                --k;
-              }
-            }
-            const uint32_t b = 8 * y + (uint32_t)__builtin_ctz((wj >> (8 * y)) & 0xFFu);
-            const uint32_t uj = __shfl(pu, j);
-            elem2[(uint64_t)bg + o] = kl.parent(hj | (K)b);
-            first_child2[(uint64_t)bg + o] = bu + uj + __popc(wj & ((1u << b) - 1u));
-          }
+  for (uint32_t r0 = 0; r0 < nz; r0 += BLOCK) {
+    const uint32_t r1 = r0 + BLOCK < nz ? r0 + BLOCK : nz;
+    const uint32_t pre0 = s_pre[r0], pre1 = r1 < nz ? s_pre[r1] : pre_end;
+    const uint32_t u0 = pre0 & 0x1FFFFu, nu = (pre1 & 0x1FFFFu) - u0;
+    const uint32_t g0 = pre0 >> 17, ng = (pre1 >> 17) - g0;
+    const bool stage = nu <= EMIT_CAP;  // the same for the whole block
+    const uint32_t it = r0 + threadIdx.x;
+    if (it < r1) {
+      const uint32_t word = s_word[it], pre = s_pre[it];
+      // high part of the word's keys: mesh and word offset within the mesh's
+      // region (the tile's first mesh unless the word lies past it; the walk
+      // keeps the region start it loaded, so no per-lane load is waited on
+      // behind this round's stores)
+      const uint64_t gw = tw0 + s_idx[it];
+      uint32_t m = m0;
+      uint64_t off = moff0;
+      if (gw >= moff1) {
+        m = m0 + 1;
+        off = moff1;
+        while (m + 1 < n_meshes) {
+          const uint64_t next = bm_off[m + 1];
+          if (gw < next) break;
+          ++m;
+          off = next;
         }
       }
+      const K hi = ((K)m << kl.cb) | ((K)(gw - off) << 5);
+      const uint32_t lu = (pre & 0x1FFFFu) - u0;
+      // unique keys: the set bits in order (two loops, so that each store
+      // names its address space)
+      if (stage) {
+        uint32_t o = lu;
+        for (uint32_t r = word; r; r &= r - 1u) s_key[o++] = hi | (K)__builtin_ctz(r);
+      } else {
+        K* out = elem1 + (uint64_t)bu + u0 + lu;
+        for (uint32_t r = word; r; r &= r - 1u) *out++ = hi | (K)__builtin_ctz(r);
+      }
+      // groups: the nonzero bytes in order; first_child2 is the unique key of
+      // the byte's lowest set bit
+      uint32_t lg = (pre >> 17) - g0;
+      const uint32_t t = (word & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+      for (uint32_t nzb = (t | word) & 0x80808080u; nzb; nzb &= nzb - 1u, ++lg) {
+        const uint32_t sh = (uint32_t)__builtin_ctz(nzb) - 7u;  // 8 x byte index
+        const uint32_t b = sh + (uint32_t)__builtin_ctz(word >> sh);
+        s_par[lg] = kl.parent(hi | (K)b);
+        s_fc[lg] = bu + (pre & 0x1FFFFu) + __popc(word & ((1u << b) - 1u));
+      }
     }
+    __syncthreads();
+    // copy-out, four elements per thread per step (LDS reads issued together)
+    if (stage) {
+      K* dst = elem1 + (uint64_t)bu + u0;
+      for (uint32_t i0 = threadIdx.x; i0 < nu; i0 += 4 * BLOCK) {
+        K v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = s_key[i0 + j * BLOCK < nu ? i0 + j * BLOCK : i0];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (i0 + j * BLOCK < nu) dst[i0 + j * BLOCK] = v[j];
+      }
+    }
+    {
+      K* dp = elem2 + (uint64_t)bg + g0;
+      uint32_t* df = first_child2 + (uint64_t)bg + g0;
+      for (uint32_t i0 = threadIdx.x; i0 < ng; i0 += 4 * BLOCK) {
+        K p[4];
+        uint32_t f[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const uint32_t i = i0 + j * BLOCK < ng ? i0 + j * BLOCK : i0;
+          p[j] = s_par[i];
+          f[j] = s_fc[i];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (i0 + j * BLOCK < ng) {
+            dp[i0 + j * BLOCK] = p[j];
+            df[i0 + j * BLOCK] = f[j];
+          }
+      }
+    }
+    __syncthreads();
   }
 }
 
