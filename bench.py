@@ -45,14 +45,15 @@ def stage_bytes(stats, name):
     if name == "sort":
         return 2 * kb * H * stats["sort_passes"]
     if name == "bitmap_clear":
-        return B
+        return stats["bitmap_fill_bytes"]
     if name == "voxelize":
         # triangles in; raw hits out (keys, or one bitmap word update each)
         return 36 * stats["n_tris"] + (4 if B else kb) * H
     if name == "unique_level1":
         # sorted hits (or the bitmap) in; unique keys, parent keys and first
-        # children out
-        return (B if B else kb * H) + kb * U + (kb + 4) * stats["n_groups1"]
+        # children out; the bitmap's nonzero 16-byte pieces zeroed
+        return ((B if B else kb * H) + kb * U + (kb + 4) * stats["n_groups1"] +
+                stats["bitmap_reset_bytes"])
     if name == "levels":
         return (kb + 4) * stats["level_elems"]
     if name == "neighbor_fill":
@@ -96,7 +97,8 @@ def pass_stats(ctx):
             "n_meshes", "n_tris", "box_cells", "n_hits", "n_voxels", "level_elems", "n_nodes",
             "n_vertices", "n_edges", "n_faces", "out_bytes")] + \
             [(k, C.c_uint32) for k in ("key_bytes", "sort_bits", "sort_passes", "n_levels")] + \
-            [("bitmap_bytes", C.c_uint64), ("n_groups1", C.c_uint64)]
+            [(k, C.c_uint64) for k in ("bitmap_bytes", "n_groups1", "bitmap_fill_bytes",
+                                       "bitmap_reset_bytes")]
     fn = L.svo_ctx_last_pass_stats
     fn.argtypes = [C.c_void_p, C.POINTER(PassStats)]
     ps = PassStats()

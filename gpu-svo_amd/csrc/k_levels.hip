@@ -220,9 +220,9 @@ __global__ __launch_bounds__(BLOCK) void k_first_pass(
 // are the parent groups (the eight children of parent p are codes 8p..8p+7,
 // one byte).  count -> scan of tile counts -> emit, each over tiles of 2048
 // words.
-constexpr int BM_WPT = 8;                 // words per thread (two 16-byte loads)
-
+constexpr int BM_WPT = 8;                 // words per thread (two 16-byte pieces)
 constexpr int BM_TILE = BLOCK * BM_WPT;   // 2048 words
+static_assert(BM_WPT % 4 == 0, "threads load whole 16-byte pieces");
 
 __device__ inline uint32_t nonzero_bytes(uint32_t w) {
   const uint32_t t = (w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;  // bit 7 of a byte: low 7 bits nonzero
@@ -232,21 +232,42 @@ __device__ inline uint32_t nonzero_bytes(uint32_t w) {
 __device__ inline void load_words(const uint32_t* bits, uint64_t n_words, uint64_t w0,
                                   uint32_t (&w)[BM_WPT]) {
   if (w0 + BM_WPT <= n_words) {
-    const uint4 a = ((const uint4*)(bits + w0))[0], b = ((const uint4*)(bits + w0))[1];
-    w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w;
-    w[4] = b.x; w[5] = b.y; w[6] = b.z; w[7] = b.w;
+#pragma unroll
+    for (int q = 0; q < BM_WPT / 4; ++q) {
+      const uint4 a = ((const uint4*)(bits + w0))[q];
+      w[4 * q] = a.x; w[4 * q + 1] = a.y; w[4 * q + 2] = a.z; w[4 * q + 3] = a.w;
+    }
   } else {
 #pragma unroll
     for (int j = 0; j < BM_WPT; ++j) w[j] = w0 + j < n_words ? bits[w0 + j] : 0u;
   }
 }
 
+// Zeroes the 16-byte pieces of the words loaded by load_words that are not
+// already zero.
+__device__ inline void clear_words(uint32_t* bits, uint64_t n_words, uint64_t w0,
+                                   const uint32_t (&w)[BM_WPT]) {
+  if (w0 + BM_WPT <= n_words) {
+#pragma unroll
+    for (int q = 0; q < BM_WPT / 4; ++q)
+      if (w[4 * q] | w[4 * q + 1] | w[4 * q + 2] | w[4 * q + 3])
+        ((uint4*)(bits + w0))[q] = make_uint4(0u, 0u, 0u, 0u);
+  } else {
+#pragma unroll
+    for (int j = 0; j < BM_WPT; ++j)
+      if (w0 + j < n_words && w[j]) bits[w0 + j] = 0u;
+  }
+}
+
 constexpr int SC_PER = 16;                  // tile counts per thread in the scan
 constexpr int SC_CHUNK = BLOCK * SC_PER;     // 4096 tiles per scan workgroup
 
+// Per tile: set bits, nonzero bytes, and the 4-byte units k_bm_emit will
+// zero.
 __global__ __launch_bounds__(BLOCK) void k_bm_count(const uint32_t* bits, uint64_t n_words,
                                                     uint32_t* tile_u, uint32_t* tile_g,
-                                                    uint64_t* scan_status, uint32_t n_status) {
+                                                    uint32_t* tile_r, uint64_t* scan_status,
+                                                    uint32_t n_status) {
   __shared__ uint64_t tmp[4];
   // the tile-count scan's look-back words and ticket start at zero
   if (blockIdx.x == 0)
@@ -254,29 +275,42 @@ __global__ __launch_bounds__(BLOCK) void k_bm_count(const uint32_t* bits, uint64
   const uint64_t w0 = (uint64_t)blockIdx.x * BM_TILE + threadIdx.x * BM_WPT;
   uint32_t w[BM_WPT];
   load_words(bits, n_words, w0, w);
-  uint32_t u = 0, g = 0;
+  // packed: set bits in 0-23, nonzero bytes in 24-43, 4-byte units zeroed by
+  // clear_words in 44-63
+  uint64_t cnt = 0;
 #pragma unroll
-  for (int j = 0; j < BM_WPT; ++j) {
-    u += __popc(w[j]);
-    g += nonzero_bytes(w[j]);
+  for (int j = 0; j < BM_WPT; ++j)
+    cnt += (uint64_t)__popc(w[j]) | ((uint64_t)nonzero_bytes(w[j]) << 24);
+  if (w0 + BM_WPT <= n_words) {
+    uint32_t units = 0;
+#pragma unroll
+    for (int q = 0; q < BM_WPT / 4; ++q)
+      units += (w[4 * q] | w[4 * q + 1] | w[4 * q + 2] | w[4 * q + 3]) ? 4u : 0u;
+    cnt += (uint64_t)units << 44;
+  } else {
+#pragma unroll
+    for (int j = 0; j < BM_WPT; ++j) cnt += (uint64_t)(w[j] != 0) << 44;
   }
   uint64_t tot;
-  block_excl_scan256(((uint64_t)g << 32) | u, tmp, tot);
+  block_excl_scan256(cnt, tmp, tot);
   if (threadIdx.x == 0) {
-    tile_u[blockIdx.x] = (uint32_t)tot;
-    tile_g[blockIdx.x] = (uint32_t)(tot >> 32);
+    tile_u[blockIdx.x] = (uint32_t)(tot & 0xFFFFFFu);
+    tile_g[blockIdx.x] = (uint32_t)((tot >> 24) & 0xFFFFFu);
+    tile_r[blockIdx.x] = (uint32_t)(tot >> 44);
   }
 }
 
 // Exclusive scan of the tile counts in place, 4096 tiles per workgroup, chunk
 // offsets by decoupled look-back (chunks claimed by ticket); totals to
 // counts[0] (unique keys = level-1 elements) and counts[1] (groups = level-2
-// elements).
+// elements); the bytes k_bm_emit will zero are added to *reset_bytes.
 __global__ __launch_bounds__(BLOCK) void k_bm_scan(uint32_t* tile_u, uint32_t* tile_g,
-                                                   uint32_t tiles, uint64_t* scan_status,
-                                                   unsigned long long* counts) {
+                                                   const uint32_t* tile_r, uint32_t tiles,
+                                                   uint64_t* scan_status,
+                                                   unsigned long long* counts,
+                                                   unsigned long long* reset_bytes) {
   __shared__ uint64_t tmp[4];
-  __shared__ uint32_t s_chunk, s_base[2];
+  __shared__ uint32_t s_chunk, s_base[2], s_reset[BLOCK / 64];
   uint32_t* ticket = (uint32_t*)scan_status;  // word 0; chunk c uses words 2 + 2c, 3 + 2c
   if (threadIdx.x == 0) s_chunk = atomicAdd(ticket, 1u);
   __syncthreads();
@@ -290,13 +324,16 @@ __global__ __launch_bounds__(BLOCK) void k_bm_scan(uint32_t* tile_u, uint32_t* t
     a[j] = tile_u[c];
     b[j] = tile_g[c];
   }
-  uint32_t su = 0, sg = 0;
+  uint32_t su = 0, sg = 0, sr = 0;
 #pragma unroll
   for (int j = 0; j < SC_PER; ++j)
     if (i0 + j < tiles) {
       su += a[j];
       sg += b[j];
+      sr += tile_r[i0 + j];
     }
+  for (int d = 32; d >= 1; d >>= 1) sr += __shfl_xor(sr, d);
+  if (__lane_id() == 0) s_reset[threadIdx.x >> 6] = sr;
   uint64_t tot;
   const uint64_t ex = block_excl_scan256(((uint64_t)sg << 32) | su, tmp, tot);
   if (threadIdx.x < 2) {
@@ -306,6 +343,11 @@ __global__ __launch_bounds__(BLOCK) void k_bm_scan(uint32_t* tile_u, uint32_t* t
     if ((uint64_t)(chunk + 1) * SC_CHUNK >= tiles) counts[threadIdx.x] = (unsigned long long)pre + c;
   }
   __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t r = 0;
+    for (int v = 0; v < BLOCK / 64; ++v) r += s_reset[v];
+    if (r) atomicAdd(reset_bytes, (unsigned long long)(4 * r));
+  }
   uint32_t ru = s_base[0] + (uint32_t)ex, rg = s_base[1] + (uint32_t)(ex >> 32);
 #pragma unroll
   for (int j = 0; j < SC_PER; ++j) {
@@ -336,11 +378,13 @@ __device__ inline uint32_t mesh_of_word(const uint64_t* bm_off, uint32_t n_meshe
 // contiguously.  A lane per nonzero word keeps most lanes busy (a lane per
 // bit would leave about four in five idle at surface densities), and staging
 // turns the lanes' strided stores into whole cache lines.  A round with more
-// than EMIT_CAP keys (dense tiles) stores its keys directly.
+// than EMIT_CAP keys (dense tiles) stores its keys directly.  The tile's
+// words are zeroed after they are read.
 constexpr uint32_t EMIT_CAP = 2048;
+static_assert((EMIT_CAP & (EMIT_CAP - 1)) == 0, "the copy-out masks slot indices");
 
 template <typename K>
-__global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_t n_words,
+__global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_words,
                                                    const uint64_t* bm_off, uint32_t n_meshes,
                                                    KeyLayout<K> kl, const uint32_t* tile_u,
                                                    const uint32_t* tile_g, K* elem1, K* elem2,
@@ -362,6 +406,7 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_
   }
   uint32_t w[BM_WPT];
   load_words(bits, n_words, tw0 + l0, w);
+  clear_words(bits, n_words, tw0 + l0, w);  // the bitmap is left clear for the next pass
   // packed counts: set bits in 0-23, nonzero bytes in 24-43, nonzero words in 44-63
   uint64_t cnt = 0;
 #pragma unroll
@@ -432,45 +477,47 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(const uint32_t* bits, uint64_
       // the byte's lowest set bit
       uint32_t lg = (pre >> 17) - g0;
       const uint32_t t = (word & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
+      // (s_fc holds the tile-relative index; the set bits below the byte's
+      // lowest are those of the lower bytes)
       for (uint32_t nzb = (t | word) & 0x80808080u; nzb; nzb &= nzb - 1u, ++lg) {
         const uint32_t sh = (uint32_t)__builtin_ctz(nzb) - 7u;  // 8 x byte index
-        const uint32_t b = sh + (uint32_t)__builtin_ctz(word >> sh);
-        s_par[lg] = kl.parent(hi | (K)b);
-        s_fc[lg] = bu + (pre & 0x1FFFFu) + __popc(word & ((1u << b) - 1u));
+        s_par[lg] = kl.parent(hi | (K)sh);
+        s_fc[lg] = (pre & 0x1FFFFu) + __popc(word & ((1u << sh) - 1u));
       }
     }
     __syncthreads();
-    // copy-out, four elements per thread per step (LDS reads issued together)
+    // copy-out, four elements per thread per step: the LDS reads are issued
+    // together (slots past the round's count read stale entries inside the
+    // buffers and store nothing)
     if (stage) {
       K* dst = elem1 + (uint64_t)bu + u0;
       for (uint32_t i0 = threadIdx.x; i0 < nu; i0 += 4 * BLOCK) {
         K v[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) v[j] = s_key[i0 + j * BLOCK < nu ? i0 + j * BLOCK : i0];
+        for (int j = 0; j < 4; ++j) v[j] = s_key[(i0 + j * BLOCK) & (EMIT_CAP - 1)];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           if (i0 + j * BLOCK < nu) dst[i0 + j * BLOCK] = v[j];
       }
     }
     {
+      // ng <= 4 * BLOCK: one step
+      const uint32_t i0 = threadIdx.x;
+      K p[4];
+      uint32_t f[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        p[j] = s_par[i0 + j * BLOCK];
+        f[j] = s_fc[i0 + j * BLOCK];
+      }
       K* dp = elem2 + (uint64_t)bg + g0;
       uint32_t* df = first_child2 + (uint64_t)bg + g0;
-      for (uint32_t i0 = threadIdx.x; i0 < ng; i0 += 4 * BLOCK) {
-        K p[4];
-        uint32_t f[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const uint32_t i = i0 + j * BLOCK < ng ? i0 + j * BLOCK : i0;
-          p[j] = s_par[i];
-          f[j] = s_fc[i];
+      for (int j = 0; j < 4; ++j)
+        if (i0 + j * BLOCK < ng) {
+          dp[i0 + j * BLOCK] = p[j];
+          df[i0 + j * BLOCK] = bu + f[j];
         }
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (i0 + j * BLOCK < ng) {
-            dp[i0 + j * BLOCK] = p[j];
-            df[i0 + j * BLOCK] = f[j];
-          }
-      }
     }
     __syncthreads();
   }
@@ -502,22 +549,43 @@ __device__ inline uint32_t wave_first_of_mesh(const K* a, uint64_t n, KeyLayout<
   return (uint32_t)(lo + (uint64_t)__popcll(__ballot(lt)));
 }
 
+// Blocks 0 .. n_meshes-1: level-1 spans of mesh m (one wave per bound) and,
+// for a mesh whose level 1 holds a single voxel, its depth-1 root (what
+// k_first_finalize does on the sort path).  Every block: zeroes the later
+// levels' look-back words and tickets, as k_level_clear does.
 template <typename K>
-__global__ __launch_bounds__(BLOCK) void k_bm_spans(const K* elem1, const K* elem2,
-                                                    const unsigned long long* counts,
-                                                    KeyLayout<K> kl, uint32_t n_meshes,
-                                                    LevelSpan* spans, uint32_t n_levels) {
+__global__ __launch_bounds__(BLOCK) void k_bm_finish(const K* elem1, const K* elem2,
+                                                     const unsigned long long* counts,
+                                                     KeyLayout<K> kl, uint32_t n_meshes,
+                                                     LevelSpan* spans, uint32_t n_levels,
+                                                     MeshTreeState* state, uint64_t* status,
+                                                     uint32_t* tickets) {
+  __shared__ uint32_t s_r[4];
+  {
+    const uint64_t words = 2 * ((counts[0] + TILE - 1) / TILE);
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < words; i += stride)
+      status[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < MAX_TICKETS) tickets[threadIdx.x] = 0;
+  }
   const uint32_t m = blockIdx.x;
+  if (m >= n_meshes) return;  // uniform over the workgroup
   const uint32_t w = threadIdx.x >> 6;  // 0: elem_first 1: elem_end 2: group_first 3: group_end
   const bool groups = w >= 2;
   const uint32_t r = wave_first_of_mesh(groups ? elem2 : elem1, groups ? counts[1] : counts[0],
                                         kl, m + (w & 1));
-  if (__lane_id() == 0) {
+  if (__lane_id() == 0) s_r[w] = r;
+  __syncthreads();
+  if (threadIdx.x == 0) {
     LevelSpan& sp = spans[(uint64_t)m * n_levels];
-    if (w == 0) sp.elem_first = r;
-    else if (w == 1) sp.elem_end = r;
-    else if (w == 2) sp.group_first = r;
-    else sp.group_end = r;
+    sp.elem_first = s_r[0];
+    sp.elem_end = s_r[1];
+    sp.group_first = s_r[2];
+    sp.group_end = s_r[3];
+    if (s_r[1] - s_r[0] == 1) {  // a single voxel: depth-1 tree
+      state[m].depth = 1;
+      state[m].root_elem = s_r[0];
+    }
   }
 }
 
@@ -687,38 +755,40 @@ void launch_level_first(const K* sorted, const unsigned long long* n_sorted, uin
 size_t bitmap_level_tmp_bytes(uint64_t n_words) {
   const uint64_t tiles = (std::max<uint64_t>(n_words, 1) + BM_TILE - 1) / BM_TILE;
   const uint64_t chunks = (tiles + SC_CHUNK - 1) / SC_CHUNK;
-  return 2 * tiles * sizeof(uint32_t) + 256 + (2 + 2 * chunks) * sizeof(uint64_t);
+  return 3 * tiles * sizeof(uint32_t) + 256 + (2 + 2 * chunks) * sizeof(uint64_t);
 }
 
 template <typename K>
-void launch_level_first_bitmap(const uint32_t* bits, uint64_t n_words, const uint64_t* bm_off,
+void launch_level_first_bitmap(uint32_t* bits, uint64_t n_words, const uint64_t* bm_off,
                                uint32_t n_meshes, KeyLayout<K> kl, LevelArrays l1,
                                LevelArrays l2, unsigned long long* counts, LevelSpan* spans1,
-                               uint32_t n_levels, void* tmp, void* lvl_tmp, hipStream_t s) {
+                               uint32_t n_levels, MeshTreeState* state, void* tmp,
+                               void* lvl_tmp, unsigned long long* reset_bytes, hipStream_t s) {
   const uint64_t tiles = (std::max<uint64_t>(n_words, 1) + BM_TILE - 1) / BM_TILE;
   const uint64_t chunks = (tiles + SC_CHUNK - 1) / SC_CHUNK;
   uint32_t* tile_u = (uint32_t*)tmp;
   uint32_t* tile_g = tile_u + tiles;
+  uint32_t* tile_r = tile_g + tiles;
   uint64_t* scan_status =
-      (uint64_t*)((char*)tmp + ((2 * tiles * sizeof(uint32_t) + 255) / 256) * 256);
+      (uint64_t*)((char*)tmp + ((3 * tiles * sizeof(uint32_t) + 255) / 256) * 256);
   const uint32_t n_status = (uint32_t)(2 + 2 * chunks);
   hipLaunchKernelGGL(k_bm_count, dim3((uint32_t)tiles), dim3(BLOCK), 0, s, bits, n_words, tile_u,
-                     tile_g, scan_status, n_status);
+                     tile_g, tile_r, scan_status, n_status);
   hipLaunchKernelGGL(k_bm_scan, dim3((uint32_t)chunks), dim3(BLOCK), 0, s, tile_u, tile_g,
-                     (uint32_t)tiles, scan_status, counts);
+                     (const uint32_t*)tile_r, (uint32_t)tiles, scan_status, counts, reset_bytes);
   hipLaunchKernelGGL(k_bm_emit<K>, dim3((uint32_t)tiles), dim3(BLOCK), 0, s, bits, n_words,
                      bm_off, n_meshes, kl, (const uint32_t*)tile_u, (const uint32_t*)tile_g,
                      (K*)l1.elem, (K*)l2.elem, l2.first_child);
-  hipLaunchKernelGGL(k_bm_spans<K>, dim3(n_meshes), dim3(BLOCK), 0, s,
-                     (const K*)l1.elem, (const K*)l2.elem, (const unsigned long long*)counts, kl,
-                     n_meshes, spans1, n_levels);
-  // the later levels' look-back words and tickets (launch_level)
+  // spans, depth-1 roots, and the later levels' look-back words and tickets
+  // (launch_level)
   uint32_t* tickets = (uint32_t*)lvl_tmp;
   uint64_t* status = (uint64_t*)((char*)lvl_tmp + 256);
   const uint64_t ltiles = (std::max<uint64_t>(l1.cap, 1) + TILE - 1) / TILE;
   const uint32_t cg = (uint32_t)std::min<uint64_t>(2048, (2 * ltiles + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL(k_level_clear, dim3(std::max<uint32_t>(cg, 1)), dim3(BLOCK), 0, s, status,
-                     tickets, (const unsigned long long*)counts);
+  hipLaunchKernelGGL(k_bm_finish<K>, dim3(std::max<uint32_t>(std::max<uint32_t>(cg, 1), n_meshes)),
+                     dim3(BLOCK), 0, s, (const K*)l1.elem, (const K*)l2.elem,
+                     (const unsigned long long*)counts, kl, n_meshes, spans1, n_levels, state,
+                     status, tickets);
 }
 
 void launch_level_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint32_t n_levels,
@@ -748,14 +818,16 @@ template void launch_level_first<uint64_t>(const uint64_t*, const unsigned long 
                                            KeyLayout<uint64_t>, LevelArrays, LevelArrays,
                                            unsigned long long*, LevelSpan*, uint32_t, uint32_t,
                                            void*, hipStream_t);
-template void launch_level_first_bitmap<uint32_t>(const uint32_t*, uint64_t, const uint64_t*,
+template void launch_level_first_bitmap<uint32_t>(uint32_t*, uint64_t, const uint64_t*,
                                                   uint32_t, KeyLayout<uint32_t>, LevelArrays,
                                                   LevelArrays, unsigned long long*, LevelSpan*,
-                                                  uint32_t, void*, void*, hipStream_t);
-template void launch_level_first_bitmap<uint64_t>(const uint32_t*, uint64_t, const uint64_t*,
+                                                  uint32_t, MeshTreeState*, void*, void*,
+                                                  unsigned long long*, hipStream_t);
+template void launch_level_first_bitmap<uint64_t>(uint32_t*, uint64_t, const uint64_t*,
                                                   uint32_t, KeyLayout<uint64_t>, LevelArrays,
                                                   LevelArrays, unsigned long long*, LevelSpan*,
-                                                  uint32_t, void*, void*, hipStream_t);
+                                                  uint32_t, MeshTreeState*, void*, void*,
+                                                  unsigned long long*, hipStream_t);
 template void launch_level<uint32_t>(int, KeyLayout<uint32_t>, LevelArrays, LevelArrays,
                                      unsigned long long*, LevelSpan*, uint32_t, MeshTreeState*,
                                      uint32_t, void*, hipStream_t);

@@ -192,6 +192,7 @@ int Workspace::get(const char* name, size_t bytes, void** out) {
     }
     b.ptr = nullptr;
     b.bytes = 0;
+    b.clean = 0;
     // grow with headroom so that slowly growing inputs do not reallocate
     const size_t want = bytes + bytes / 8;
     void* p = nullptr;
@@ -325,6 +326,8 @@ struct CallTiming {
       ps.out_bytes += p.out_bytes;
       ps.bitmap_bytes += p.bitmap_bytes;
       ps.n_groups1 += p.n_groups1;
+      ps.bitmap_fill_bytes += p.bitmap_fill_bytes;
+      ps.bitmap_reset_bytes += p.bitmap_reset_bytes;
       ps.key_bytes = std::max(ps.key_bytes, p.key_bytes);
       ps.sort_bits = std::max(ps.sort_bits, p.sort_bits);
       ps.sort_passes = std::max(ps.sort_passes, p.sort_passes);
@@ -496,7 +499,8 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   MeshParams* d_par = (MeshParams*)(ctl + sz_md + sz_tb + sz_bo + sz_acc);
   unsigned long long* d_cnt =
       (unsigned long long*)(ctl + sz_md + sz_tb + sz_bo + sz_acc + sz_par);
-  // d_cnt: [0] box cells, [1] big chunks, [2] keys emitted, [3] big queued
+  // d_cnt: [0] box cells, [1] big chunks, [2] keys emitted, [3] big queued,
+  // [4] bitmap bytes zeroed by the level-1 pass
   SVO_TRY(L.ensure_pinned(sz_md + sz_tb + sz_bo + 64 + sizeof(MeshParams) * n));
   {
     // stage the mesh descriptors, triangle bases and bitmap offsets through
@@ -612,16 +616,28 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
 
   // ---- A1: voxelize, sort, levels
   timer.mark("setup_a1");
+  // The level-1 pass zeroes the bitmap words it reads, so the lane's bitmap
+  // stays clear between passes; only words never cleared since the buffer
+  // was allocated are filled here (the stage is absent otherwise).  Until
+  // the level-1 pass is launched the bitmap counts as dirty.
+  size_t bm_clean_after = 0, bm_fill = 0;
   if (use_bm) {
-    launch_fill_u32(bitmap, 0u, bm_words, s);
-    timer.mark("bitmap_clear");
+    size_t& clean = W.clean_bytes("bitmap");
+    if (clean < bm_bytes) {
+      bm_fill = bm_bytes - clean;
+      launch_fill_u32(bitmap + clean / 4, 0u, bm_fill / 4, s);
+      timer.mark("bitmap_clear");
+    }
+    bm_clean_after = std::max(clean, (size_t)bm_bytes);
+    clean = 0;
   }
   launch_voxelize<K>(d_md, d_par, d_tb, n, T, plan.cb, tn, keys_a, bitmap, d_bo,
                      d_cnt + 2, big, d_cnt + 3, n_big, s);
   timer.mark("voxelize");
   if (use_bm) {
     launch_level_first_bitmap<K>(bitmap, bm_words, d_bo, n, kl, lv[1], lv[2], d_counts, d_spans,
-                                 NL, bm_tmp, lvl_tmp, s);
+                                 NL, d_state, bm_tmp, lvl_tmp, d_cnt + 4, s);
+    W.clean_bytes("bitmap") = bm_clean_after;
   } else {
     K* sorted = nullptr;
     launch_radix_sort<K>(keys_a, keys_b, d_cnt + 2, key_cap, plan.cb + plan.mb,
@@ -629,8 +645,8 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     timer.mark("sort");
     launch_level_first<K>(sorted, d_cnt + 2, key_cap, kl, lv[1], lv[2], d_counts, d_spans, NL,
                           C.tuning.scan_blocks, lvl_tmp, s);
+    launch_level_first_finalize(d_spans, n, NL, d_state, s);
   }
-  launch_level_first_finalize(d_spans, n, NL, d_state, s);
   timer.mark("unique_level1");
   for (uint32_t l = 2; l <= NL; ++l)
     launch_level<K>((int)l, kl, lv[l], lv[l + 1], d_counts, d_spans, NL, d_state,
@@ -651,9 +667,9 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   HIP_TRY(hipMemcpyAsync(h_spans, d_spans, sizeof(LevelSpan) * n * NL, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(h_state, d_state, sizeof(MeshTreeState) * n, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(h_par, d_par, sizeof(MeshParams) * n, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(h_hits, d_cnt + 2, 8, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(h_hits, d_cnt + 2, 24, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  const uint64_t n_hits = *h_hits;
+  const uint64_t n_hits = h_hits[0], bm_reset = h_hits[2];
   if (n_hits > key_cap) return SVO_E_HIP;  // cannot happen: hits <= box cells
   // the pinned area is reused below: keep host copies of the readback
   const std::vector<uint64_t> v_counts(h_counts, h_counts + NL + 2);
@@ -815,6 +831,8 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     ps.sort_passes = use_bm ? 0 : (ps.sort_bits + 7) / 8;
     ps.bitmap_bytes = use_bm ? bm_bytes : 0;
     ps.n_groups1 = h_counts[1];
+    ps.bitmap_fill_bytes = bm_fill;
+    ps.bitmap_reset_bytes = bm_reset;
     ps.n_levels = NL;
     timer.publish(*plan.timing, ps);
   }

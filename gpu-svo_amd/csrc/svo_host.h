@@ -23,10 +23,13 @@ struct Workspace {
   struct Buf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    size_t clean = 0;  // leading bytes known to be zero (kept by the caller)
   };
   Context* ctx = nullptr;
   std::unordered_map<std::string, Buf> bufs;
   int get(const char* name, size_t bytes, void** out);
+  // Leading bytes of buffer `name` known to be zero; 0 after (re)allocation.
+  size_t& clean_bytes(const char* name) { return bufs[name].clean; }
   void release_all();
   size_t total() const;
 };

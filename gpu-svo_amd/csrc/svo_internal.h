@@ -77,16 +77,20 @@ void launch_level_first(const K* sorted, const unsigned long long* n_sorted, uin
                         unsigned long long* counts, LevelSpan* spans1, uint32_t n_levels,
                         uint32_t blocks, void* tmp, hipStream_t s);
 // Level 1 from the occupancy bitmap (n_words words; mesh m owns words
-// [bm_off[m], bm_off[m+1])): the same outputs as launch_level_first.  `tmp`
-// needs bitmap_level_tmp_bytes(n_words); `lvl_tmp` (level_tmp_bytes of
-// l1.cap) is prepared for the later levels.
+// [bm_off[m], bm_off[m+1])): the same outputs as launch_level_first followed
+// by launch_level_first_finalize, and the bitmap is left all zero (the bytes
+// that takes are added to *reset_bytes).  `tmp` needs
+// bitmap_level_tmp_bytes(n_words); `lvl_tmp` (level_tmp_bytes of l1.cap) is
+// prepared for the later levels.
 size_t bitmap_level_tmp_bytes(uint64_t n_words);
 template <typename K>
-void launch_level_first_bitmap(const uint32_t* bits, uint64_t n_words, const uint64_t* bm_off,
+void launch_level_first_bitmap(uint32_t* bits, uint64_t n_words, const uint64_t* bm_off,
                                uint32_t n_meshes, KeyLayout<K> kl, LevelArrays l1,
                                LevelArrays l2, unsigned long long* counts, LevelSpan* spans1,
-                               uint32_t n_levels, void* tmp, void* lvl_tmp, hipStream_t s);
-// Marks meshes whose level-1 set holds a single voxel as depth-1 trees.
+                               uint32_t n_levels, MeshTreeState* state, void* tmp,
+                               void* lvl_tmp, unsigned long long* reset_bytes, hipStream_t s);
+// Marks meshes whose level-1 set holds a single voxel as depth-1 trees (the
+// sort path; launch_level_first_bitmap does it itself).
 void launch_level_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint32_t n_levels,
                                  MeshTreeState* state, hipStream_t s);
 // One level l >= 2: roots of meshes whose level holds one element; groups of

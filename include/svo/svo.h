@@ -188,6 +188,9 @@ typedef struct {
   uint32_t key_bytes, sort_bits, sort_passes, n_levels;
   uint64_t bitmap_bytes; /* occupancy bitmap of the pass; 0 when the hits were sorted */
   uint64_t n_groups1;    /* parent groups of the unique voxels (level-2 elements) */
+  uint64_t bitmap_fill_bytes;  /* bitmap bytes cleared before voxelization (0 when the
+                                  lane's bitmap was already clear) */
+  uint64_t bitmap_reset_bytes; /* bitmap bytes zeroed by the level-1 pass after reading */
 } svo_pass_stats;
 int svo_ctx_last_pass_stats(svo_ctx* ctx, svo_pass_stats* out);
 uint64_t svo_ctx_bytes_in_use(const svo_ctx* ctx);

@@ -144,7 +144,10 @@ def test_obj_files_byte_exact(gpu_ctx, bunny, tmp_path, res, kb):
 @pytest.mark.parametrize("path", [1, 2], ids=["bitmap", "sort"])
 def test_dedup_paths(gpu_ctx, bunny, path):
     """Both de-duplication paths (occupancy bitmap, radix sort of the hits)
-    give the oracle's tree."""
+    give the oracle's tree.  The builds share one context, so the bitmap
+    path reuses one bitmap buffer at sizes that grow and shrink: a bit left
+    set by an earlier build (the level-1 pass clears what it reads) would
+    show as an extra voxel."""
     gpu_ctx.set_option("dedup_path", path)
     assert gpu_ctx.get_option("dedup_path") == path
     try:

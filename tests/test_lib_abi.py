@@ -57,7 +57,7 @@ def test_struct_layouts_match_c(tmp_path):
             svo_amd.VERT_DTYPE.itemsize, svo_amd.EDGE_DTYPE.itemsize,
             svo_amd.NODE_DTYPE.itemsize, svo_amd.TreeInfo.level_nodes.offset,
             svo_amd.NODE_DTYPE.fields["neighbors"][1], C.sizeof(svo_amd.TuneResult),
-            11 * 8 + 4 * 4 + 2 * 8]
+            11 * 8 + 4 * 4 + 4 * 8]
     assert got == want
     assert got[5:8] == [16, 28, 168]
 
