@@ -156,6 +156,9 @@ def run_batch_workload(args):
     ctx.set_timing(True)
     if args.batch_lanes:
         ctx.set_option("batch_lanes", args.batch_lanes)
+    for opt in args.option:
+        name, value = opt.split("=", 1)
+        ctx.set_option(name, int(value))
     dms = [ctx.upload_mesh(P, T) for P, T, _ in meshes]
     descs = [dm.desc(res) for dm, (_, _, res) in zip(dms, meshes)]
     tuning = None
@@ -238,6 +241,7 @@ def run_batch_workload(args):
         "sizes": dict({"voxels": nv, "nodes": nn, "hits": stats["n_hits"]}, **sizes),
         "passes": ctx.get_option("batch_lanes"),
         "tuning": tuning,
+        "options": {o.split("=", 1)[0]: int(o.split("=", 1)[1]) for o in args.option} or None,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "dedup_gbs": dedup_gbs(stats, stages),
         "dedup_path": "bitmap" if stats["bitmap_bytes"] else "sort",
@@ -515,6 +519,8 @@ def main():
                     help="batch workload: run the library's tuning search on the batch first")
     ap.add_argument("--batch-lanes", type=int, default=0,
                     help="concurrent passes a batch is split into (0: library default)")
+    ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
+                    help="batch workload: a library option (svo_ctx_set_option) to set first")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first copy this many MiB device-to-device once: a kernel of known "
                          "traffic for scaling HBM counters")
