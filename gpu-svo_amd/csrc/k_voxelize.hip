@@ -49,6 +49,19 @@ __device__ inline uint32_t find_mesh(const uint64_t* base, uint32_t n_meshes, ui
   return lo;
 }
 
+// Morton code stepping: the bits of one axis are a dilated integer, so adding
+// one to that axis is an add through the other axes' bits (set to one so the
+// carry passes them) and a merge back.  x is bits 0, 3, 6, ...; y and z the
+// same shifted by one and two.
+template <typename K>
+__device__ inline K morton_x_mask() {
+  return sizeof(K) == 4 ? (K)0x09249249u : (K)0x1249249249249249ull;
+}
+template <typename K>
+__device__ inline K dilated_inc(K code, K mask) {
+  return (((code | ~mask) + (K)1) & mask) | (code & ~mask);
+}
+
 // ------------------------------------------------------------------ bbox
 __global__ void k_bbox_init(uint32_t* acc, uint32_t n_meshes) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -495,6 +508,16 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
   int x = S.g0[0], y = S.g0[1], z = S.g0[2];
   bool row_ok = false;
   RowTerms R;
+  // the Morton code of (x, y, z) as an x part (back to mx0 at each row start)
+  // and a y|z part stepped row by row
+  const K MX = morton_x_mask<K>(), MY = MX << 1, MZ = MX << 2;
+  K mx0 = 0, my0 = 0, cyz = 0;
+  if (count) {
+    mx0 = morton3<K>((uint32_t)S.g0[0], 0u, 0u);
+    my0 = morton3<K>(0u, (uint32_t)S.g0[1], 0u);
+    cyz = my0 | morton3<K>(0u, 0u, (uint32_t)S.g0[2]);
+  }
+  K cx = mx0;
   while (__any(c < count)) {
     bool hit = false;
     K code = 0;
@@ -502,17 +525,22 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
       if (x == S.g0[0]) row_ok = row_terms(S, y, z, R);
       if (row_ok) {
         hit = cell_hit(S, R, x);
-        code = morton3<K>((uint32_t)x, (uint32_t)y, (uint32_t)z);
+        code = cx | cyz;
       }
       if (row_ok && x < S.g1[0]) {
         ++x;
         ++c;
+        cx = dilated_inc(cx, MX);
       } else {  // end of row, or an empty row skipped as a whole
         c += (uint64_t)(S.g1[0] - x + 1);
         x = S.g0[0];
+        cx = mx0;
         if (++y > S.g1[1]) {
           y = S.g0[1];
           ++z;
+          cyz = dilated_inc((cyz & ~MY) | my0, MZ);
+        } else {
+          cyz = dilated_inc(cyz, MY);
         }
       }
     }
@@ -569,11 +597,14 @@ __global__ __launch_bounds__(BLOCK) void k_vox_big(const MeshDev* meshes,
         z = S.g0[2] + (int)(r / ny);
         if (row_terms(S, y, z, R)) row_range(S, RC, R, xa, xb);
       }
+      // the code steps with x; past xb it is not used
+      K code = morton3<K>((uint32_t)xa, (uint32_t)y, (uint32_t)z);
+      const K MX = morton_x_mask<K>();
       for (int x = xa; __any(x <= xb); ++x) {
         const bool hit = (x <= xb) & cell_hit(S, R, x);
-        const K code = morton3<K>((uint32_t)x, (uint32_t)y, (uint32_t)z);
         if (BITMAP) bs.push(hit, wbase + (uint64_t)(code >> 5), (uint32_t)(code & 31));
         else st.push(hit, mesh_key | code);
+        code = dilated_inc(code, MX);
       }
     }
   }
