@@ -90,6 +90,29 @@ def test_key64(gpu_ctx):
     _case(gpu_ctx, P, T, 4096, kb=64, sem=1)
 
 
+def test_dense_bitmap_tiles(gpu_ctx):
+    """Most cells occupied: a round of the level-1 emission (256 nonzero
+    bitmap words) then holds more unique keys than its LDS stage (2048, i.e.
+    more than eight per word), and its keys are stored directly."""
+    # 32 stacked unit squares (two triangles each), one per 1/32 in z: at
+    # res 32 every cell is a voxel; at res 48 most are
+    P, T = [], []
+    for k in range(32):
+        z = (k + 0.5) / 32
+        P += [[0, 0, z], [1, 0, z], [1, 1, z], [0, 1, z]]
+        T += [[4 * k, 4 * k + 1, 4 * k + 2], [4 * k, 4 * k + 2, 4 * k + 3]]
+    P, T = np.array(P, np.float32), np.array(T, np.int32)
+    for path in (1, 2):  # the bitmap path, and the sort path on the same input
+        gpu_ctx.set_option("dedup_path", path)
+        try:
+            for res in (32, 48):
+                tree, _ = _case(gpu_ctx, P, T, res)
+                assert tree.info.n_voxels > 0.5 * res ** 3, "case too sparse to fill the stage"
+                tree.free()
+        finally:
+            gpu_ctx.set_option("dedup_path", 0)
+
+
 def test_single_triangle(gpu_ctx):
     P = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0.5]], np.float32)
     T = np.array([[0, 1, 2]], np.int32)
