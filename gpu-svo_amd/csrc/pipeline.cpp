@@ -61,9 +61,13 @@ void Context::destroy() {
   for (auto& l : lanes) {
     if (l->stream) hipStreamSynchronize(l->stream);
     l->ws.release_all();
+    if (l->side) hipStreamSynchronize(l->side);
     for (auto e : l->events) hipEventDestroy(e);
+    if (l->fork) hipEventDestroy(l->fork);
+    if (l->join) hipEventDestroy(l->join);
     if (l->pinned) hipHostFree(l->pinned);
     if (l->stream) hipStreamDestroy(l->stream);
+    if (l->side) hipStreamDestroy(l->side);
   }
   lanes.clear();
   free_lanes.clear();
@@ -77,7 +81,16 @@ Lane* Context::acquire_lane() {
   if (free_lanes.empty() && lanes.size() < 8) {
     auto l = std::make_unique<Lane>();
     hipSetDevice(device);
-    if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&l->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&l->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&l->join, hipEventDisableTiming) != hipSuccess) {
+      if (l->stream) hipStreamDestroy(l->stream);
+      if (l->side) hipStreamDestroy(l->side);
+      if (l->fork) hipEventDestroy(l->fork);
+      if (l->join) hipEventDestroy(l->join);
+      return nullptr;
+    }
     l->ws.ctx = this;
     free_lanes.push_back(l.get());
     lanes.push_back(std::move(l));
@@ -352,6 +365,14 @@ struct StageTimer {
   Lane& L;
   std::vector<std::string> names;
   size_t used = 0;
+  // stages timed on the side stream: name, start event, end event (indices
+  // into L.events past the main-stream marks)
+  struct Range {
+    std::string name;
+    size_t a, b;
+  };
+  std::vector<Range> ranges;
+  std::vector<hipEvent_t> side_ev;
   StageTimer(Context& c, Lane& l) : C(c), L(l) {}
   void mark(const char* name) {
     if (!C.timing) return;
@@ -363,6 +384,24 @@ struct StageTimer {
     hipEventRecord(L.events[used], L.stream);
     names.push_back(name);
     ++used;
+  }
+  // Records a timing event on `st` (the side stream); returns its index.
+  size_t side_mark(hipStream_t st) {
+    if (!C.timing) return 0;
+    const size_t i = side_ev.size();
+    // side events live in L.events after a fixed block of 64 main marks
+    const size_t slot = 64 + i;
+    while (L.events.size() <= slot) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return 0;
+      L.events.push_back(e);
+    }
+    hipEventRecord(L.events[slot], st);
+    side_ev.push_back(L.events[slot]);
+    return i;
+  }
+  void side_range(const char* name, size_t a, size_t b) {
+    if (C.timing) ranges.push_back({name, a, b});
   }
   // Hands the pass's stage windows and sizes to the call's figures.
   void publish(CallTiming& ct, const svo_pass_stats& ps) {
@@ -376,6 +415,15 @@ struct StageTimer {
         a.push_back(x);
         b.push_back(y);
         nm.push_back(names[i]);
+      }
+    if (C.timing && ct.ref)
+      for (const Range& r : ranges) {
+        float x = 0, y = 0;
+        hipEventElapsedTime(&x, ct.ref, side_ev[r.a]);
+        hipEventElapsedTime(&y, ct.ref, side_ev[r.b]);
+        a.push_back(x);
+        b.push_back(y);
+        nm.push_back(r.name);
       }
     ct.add(nm, a, b, ps);
   }
@@ -773,12 +821,29 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   HIP_TRY(hipMemcpyAsync(bctl, L.pinned, sz_lay + sz_nb, hipMemcpyHostToDevice, s));
   launch_fill_u32((uint32_t*)d_tot, 0u, sz_tot / 4, s);
   timer.mark("setup_b");
-  launch_fill_u32(na.neighbors, NONE, 27 * N, s);
-  timer.mark("neighbor_fill");
+  // When this pass runs alone on the context, the neighbor fill (a pure
+  // write stream) runs on the side stream beside the node records (bound by
+  // their loads' latency); the roots' neighbor slot is written after the
+  // join.  Concurrent passes already overlap each other, and their extra
+  // streams would outnumber the hardware queues, so they fill in order.
+  const bool side_fill = C.active_passes.load() == 1;
+  if (side_fill) {
+    HIP_TRY(hipEventRecord(L.fork, s));
+    HIP_TRY(hipStreamWaitEvent(L.side, L.fork, 0));
+    const size_t e0 = timer.side_mark(L.side);
+    launch_fill_u32(na.neighbors, NONE, 27 * N, L.side);
+    const size_t e1 = timer.side_mark(L.side);
+    timer.side_range("neighbor_fill", e0, e1);
+    HIP_TRY(hipEventRecord(L.join, L.side));
+  } else {
+    launch_fill_u32(na.neighbors, NONE, 27 * N, s);
+    timer.mark("neighbor_fill");
+  }
   for (uint32_t l = 1; l < NL; ++l)
     launch_emit_level_nodes<K>((int)l, kl, lv[l].elem, h_counts[l - 1], lv[l + 1].elem,
                                lv[l + 1].elem_group, lv[l + 1].first_child, h_counts[l], d_spans,
                                NL, d_state, d_lay, d_par, na, s);
+  if (side_fill) HIP_TRY(hipStreamWaitEvent(s, L.join, 0));
   {
     std::vector<void*> lp(NL + 1, nullptr);
     for (uint32_t l = 1; l <= NL; ++l) lp[l - 1] = lv[l].elem;
@@ -899,6 +964,11 @@ namespace {
 // One pipeline pass over the valid meshes `sel` of `in` on one lane.
 int run_pass(Context& C, const std::vector<MeshInput>& in, const std::vector<uint32_t>& sel,
              std::vector<svo_tree*>& trees, std::vector<int>& statuses, CallTiming& ct) {
+  struct ActivePass {
+    Context& c;
+    explicit ActivePass(Context& x) : c(x) { c.active_passes.fetch_add(1); }
+    ~ActivePass() { c.active_passes.fetch_sub(1); }
+  } active(C);
   Plan plan;
   plan.timing = &ct;
   bool any64 = false;

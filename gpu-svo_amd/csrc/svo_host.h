@@ -34,9 +34,13 @@ struct Workspace {
   size_t total() const;
 };
 
-// One stream and its workspace; one build runs on a lane at a time.
+// One stream and its workspace; one build runs on a lane at a time.  `side`
+// runs work of the pass that overlaps the main stream (forked and joined
+// with `fork` / `join`).
 struct Lane {
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   Workspace ws;
   void* pinned = nullptr;  // host readback area
   size_t pinned_bytes = 0;
@@ -67,6 +71,7 @@ struct Context {
   uint64_t budget = 0;  // 0 = unbounded
   std::atomic<uint64_t> in_use{0};
   std::atomic<uint64_t> ws_bytes{0};  // part of in_use held by lane workspaces (reused)
+  std::atomic<int> active_passes{0};  // pipeline passes running on the context
   Tuning tuning;
   std::mutex mu;
   std::condition_variable cv;
