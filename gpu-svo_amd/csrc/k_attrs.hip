@@ -135,7 +135,7 @@ __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
   const uint32_t k = (uint32_t)(threadIdx.x & 7);
   NodeIn<K> a[NODES_PT];
 #pragma unroll
-  for (int u = 0; u < NODES_PT; ++u) {  // round 1
+  for (int u = 0; u < NODES_PT; ++u) {  // first loads: the parent group
     const uint64_t g0 = (base + (uint64_t)u * BLOCK) >> 3;
     a[u].in = g0 < n_groups;
     a[u].g = a[u].in ? g0 : 0;
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
     q.oz = params[m].origin[2];
   };
 #pragma unroll
-  for (int u = 0; u < NODES_PT; ++u) {  // round 2
+  for (int u = 0; u < NODES_PT; ++u) {  // second loads: mesh records, child key
     const uint32_t m = kl.mesh(a[u].P);
     // a wave's nodes nearly always share one mesh: its per-mesh records are
     // then read once for the wave (scalar loads) instead of once per lane
