@@ -145,11 +145,15 @@ __global__ __launch_bounds__(BLOCK) void k_first_pass(
     uint64_t tot;
     const uint64_t ex = block_excl_scan256(((uint64_t)ng << 32) | nu, tmp, tot);
     const uint32_t tu = (uint32_t)tot, tg = (uint32_t)(tot >> 32);
-    if (threadIdx.x < 2) {
-      const uint32_t c = threadIdx.x == 0 ? tu : tg;
-      const uint32_t pre = lookback_exclusive(status + threadIdx.x, 2, r.tile, c, epoch);
-      s_base[threadIdx.x] = pre;
-      if (r.tb + r.tn == n) counts[threadIdx.x] = (unsigned long long)pre + c;
+    if (threadIdx.x < 128) {
+      // wave 0: unique-key offset, wave 1: group offset
+      const uint32_t wv = threadIdx.x >> 6;
+      const uint32_t c = wv == 0 ? tu : tg;
+      const uint32_t pre = lookback_exclusive_wave(status + wv, 2, r.tile, c, epoch);
+      if (__lane_id() == 0) {
+        s_base[wv] = pre;
+        if (r.tb + r.tn == n) counts[wv] = (unsigned long long)pre + c;
+      }
     }
     __syncthreads();
     const uint32_t bu = s_base[0], bg = s_base[1];
@@ -653,10 +657,12 @@ __global__ __launch_bounds__(BLOCK) void k_level_pass(
     }
     uint32_t tg;
     const uint32_t exg = block_excl_scan256(ng, tmp, tg);
-    if (threadIdx.x == 0) {
-      const uint32_t pre = lookback_exclusive(status + 1, 2, r.tile, tg, epoch);
-      s_base = pre;
-      if (r.tb + r.tn == n) *count_next = (unsigned long long)pre + tg;
+    if (threadIdx.x < 64) {
+      const uint32_t pre = lookback_exclusive_wave(status + 1, 2, r.tile, tg, epoch);
+      if (threadIdx.x == 0) {
+        s_base = pre;
+        if (r.tb + r.tn == n) *count_next = (unsigned long long)pre + tg;
+      }
     }
     __syncthreads();
     const uint32_t bg = s_base;

@@ -132,4 +132,63 @@ __device__ inline uint32_t lookback_exclusive(uint64_t* words, uint64_t stride, 
   return prefix;
 }
 
+// The same with a whole wave: all 64 lanes call it with the same arguments
+// and each lane loads one earlier word per step, so a step covers 64 tiles.
+// A single lane covering LB_WINDOW words per memory round trip caps how many
+// tiles a pass can finish per microsecond; passes over thousands of short
+// tiles run into that cap.  Returns the prefix in every lane.
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+__device__ inline uint32_t lookback_exclusive_wave(uint64_t* words, uint64_t stride,
+                                                   uint64_t tile, uint32_t count,
+                                                   uint32_t epoch) {
+  epoch &= 0x3FFFFFFFu;
+  const uint32_t lane = __lane_id();
+  uint64_t* own = words + tile * stride;
+  if (tile == 0) {
+    if (lane == 0)
+      __hip_atomic_store(own, lb_word(LB_INCL, epoch, count), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  if (lane == 0)
+    __hip_atomic_store(own, lb_word(LB_AGG, epoch, count), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t prefix = 0;
+  int64_t t = (int64_t)tile - 1;  // the same in every lane
+  while (true) {
+    const int64_t ti = t - (int64_t)lane;
+    // lanes past tile 0 read nothing: tile 0's inclusive word ends the walk
+    // before them
+    const uint64_t x = ti >= 0 ? __hip_atomic_load(words + (uint64_t)ti * stride,
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                               : 0ull;
+    const bool valid = ti >= 0 && (x & LB_FLAGS) != 0 &&
+                       (uint32_t)((x >> 32) & 0x3FFFFFFFu) == epoch;
+    const uint64_t bad = __ballot(!valid), incl = __ballot(valid && (x & LB_FLAGS) == LB_INCL);
+    const uint32_t first_bad = bad ? (uint32_t)__builtin_ctzll(bad) : 64u;
+    const uint32_t first_incl = incl ? (uint32_t)__builtin_ctzll(incl) : 64u;
+    if (first_incl < first_bad) {
+      // every word up to the nearest inclusive one is published
+      prefix += wave_sum_u32(lane <= first_incl ? (uint32_t)x : 0u);
+      break;
+    }
+    if (first_bad > 0) {
+      // the published aggregates before the first unpublished word
+      prefix += wave_sum_u32(lane < first_bad ? (uint32_t)x : 0u);
+      t -= first_bad;  // tile 0 publishes an inclusive word, so t stays >= 0
+    } else {
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  if (lane == 0)
+    __hip_atomic_store(own, lb_word(LB_INCL, epoch, prefix + count), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  return prefix;
+}
+
 }  // namespace svo
