@@ -826,15 +826,22 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   // their loads' latency); the roots' neighbor slot is written after the
   // join.  Concurrent passes already overlap each other, and their extra
   // streams would outnumber the hardware queues, so they fill in order.
-  const bool side_fill = C.active_passes.load() == 1;
+  bool side_fill = C.active_passes.load() == 1;
+  if (side_fill && (hipEventRecord(L.fork, s) != hipSuccess ||
+                    hipStreamWaitEvent(L.side, L.fork, 0) != hipSuccess)) {
+    hipGetLastError();
+    side_fill = false;  // fill in order instead
+  }
   if (side_fill) {
-    HIP_TRY(hipEventRecord(L.fork, s));
-    HIP_TRY(hipStreamWaitEvent(L.side, L.fork, 0));
     const size_t e0 = timer.side_mark(L.side);
     launch_fill_u32(na.neighbors, NONE, 27 * N, L.side);
     const size_t e1 = timer.side_mark(L.side);
     timer.side_range("neighbor_fill", e0, e1);
-    HIP_TRY(hipEventRecord(L.join, L.side));
+    if (hipEventRecord(L.join, L.side) != hipSuccess) {
+      hipGetLastError();
+      hipStreamSynchronize(L.side);  // no join event: wait for the fill here
+      side_fill = false;
+    }
   } else {
     launch_fill_u32(na.neighbors, NONE, 27 * N, s);
     timer.mark("neighbor_fill");
@@ -843,7 +850,10 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     launch_emit_level_nodes<K>((int)l, kl, lv[l].elem, h_counts[l - 1], lv[l + 1].elem,
                                lv[l + 1].elem_group, lv[l + 1].first_child, h_counts[l], d_spans,
                                NL, d_state, d_lay, d_par, na, s);
-  if (side_fill) HIP_TRY(hipStreamWaitEvent(s, L.join, 0));
+  if (side_fill && hipStreamWaitEvent(s, L.join, 0) != hipSuccess) {
+    hipStreamSynchronize(L.side);  // the fill writes the output block: let it end first
+    return SVO_E_HIP;
+  }
   {
     std::vector<void*> lp(NL + 1, nullptr);
     for (uint32_t l = 1; l <= NL; ++l) lp[l - 1] = lv[l].elem;
