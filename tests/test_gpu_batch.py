@@ -84,6 +84,27 @@ def test_batch_mixed_status(gpu_ctx):
             t.free()
 
 
+@pytest.mark.parametrize("path", [1, 2], ids=["bitmap", "sort"])
+def test_batch_odd_resolutions(gpu_ctx, path):
+    """Resolutions whose bitmap regions are no whole number of 16-byte pieces
+    or level-1 tiles: mesh boundaries fall at odd word offsets inside tiles."""
+    from svo_amd import meshgen
+    P3, T3 = meshgen.icosphere(3)
+    P4, T4 = meshgen.icosphere(4)
+    meshes = [(P3, T3, 5), (P4, T4, 17), (P3, T3, 33), (P4, T4, 47), (P3, T3, 100),
+              (P4, T4, (40, 24, 33)), (P3, T3, 3), (P4, T4, 129)]
+    gpu_ctx.set_option("dedup_path", path)
+    try:
+        trees, st = gpu_ctx.build_batch(meshes)
+    finally:
+        gpu_ctx.set_option("dedup_path", 0)
+    assert all(s == 0 for s in st), st
+    for m, t in zip(meshes, trees):
+        print(f"mesh res {m[2]}", flush=True)
+        compare(t, po.build(*m))
+        t.free()
+
+
 def test_batch_single_voxel_meshes(gpu_ctx):
     # meshes whose voxel set is a single cell (depth-1 trees) next to normal ones
     tri = np.array([[0, 1, 2]], np.int32)
