@@ -277,6 +277,9 @@ def run_single_workload(args, name, P, T, res, key_bits, flags=0, desc=""):
     """One large mesh per step (plain path), inputs resident in HBM."""
     import svo_amd
     ctx = svo_amd.Context(0)
+    for opt in args.option:
+        oname, value = opt.split("=", 1)
+        ctx.set_option(oname, int(value))
     ctx.set_timing(True)
     dm = ctx.upload_mesh(P, T)
     stage_acc: dict = {}
@@ -311,6 +314,7 @@ def run_single_workload(args, name, P, T, res, key_bits, flags=0, desc=""):
                   "box_cells": stats["box_cells"], "out_gb": round(stats["out_bytes"] / 1e9, 3)},
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "dedup_gbs": dedup_gbs(stats, stages),
+        "options": {o.split("=", 1)[0]: int(o.split("=", 1)[1]) for o in args.option} or None,
         "dedup_path": "bitmap" if stats["bitmap_bytes"] else "sort",
         "roofline": _roofline(stats, stages),
     }
@@ -520,7 +524,8 @@ def main():
     ap.add_argument("--batch-lanes", type=int, default=0,
                     help="concurrent passes a batch is split into (0: library default)")
     ap.add_argument("--option", action="append", default=[], metavar="NAME=VALUE",
-                    help="batch workload: a library option (svo_ctx_set_option) to set first")
+                    help="batch and single-mesh workloads: a library option "
+                         "(svo_ctx_set_option) to set first")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first copy this many MiB device-to-device once: a kernel of known "
                          "traffic for scaling HBM counters")

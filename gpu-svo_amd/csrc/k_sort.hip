@@ -1,5 +1,5 @@
 // k_sort.hip -- LSD radix sort of voxel keys: 8-bit digits, one read for all
-// digit histograms, then one kernel per digit pass (onesweep).
+// digit histograms, then three kernels per digit pass (reduce, scan, scatter).
 //
 // Replaces the reference's dense flag array + exclusive scan + compaction
 // (SVO.cu:367-405), which is O(res^3), with a sort of the emitted hit keys,
@@ -8,18 +8,22 @@
 //
 // The key count n is read from device memory, so no launch shape depends on
 // the hit count.  Kernels:
-//   k_sort_clear  zero the histograms, tile tickets and the status words of
-//                 the tiles the n keys occupy
+//   k_sort_clear  zero the histograms
 //   k_upsweep     digit histograms of every pass in one read of the keys
 //   k_scan_hist   exclusive scan of each pass's histogram -> digit offsets
-//   k_onesweep    per pass: persistent blocks claim 4096-key tiles by atomic
-//                 ticket, rank the keys stably with wave64 match masks
-//                 (8 ballots per 64 keys), publish per-digit tile counts and
-//                 find their exclusive prefix over earlier tiles by
-//                 decoupled look-back, then write through an LDS reorder so
-//                 the stores are contiguous within each digit run.
+//   per pass:
+//   k_tile_hist   digit counts of each 4096-key tile, stored digit-major
+//   k_tile_scan   per digit, exclusive scan over the tiles plus the digit's
+//                 offset -> where each tile's run of each digit starts
+//   k_scatter     rank the tile's keys stably with wave64 match masks (8
+//                 ballots per 64 keys) and write them through an LDS reorder
+//                 so the stores are contiguous within each digit run
 // Stable, so the passes compose into a full LSD sort.  HBM traffic for n keys
-// of B bytes and P passes: n*B (upsweep) + P * 2*n*B.
+// of B bytes and P passes: n*B (upsweep) + P * 3*n*B, plus 2 KB of counts per
+// tile and pass.  A single-pass scatter with decoupled look-back reads and
+// writes the keys once per pass, but over thousands of tiles its look-back
+// chain (and the 256 digits' status words each tile walks) bounds the pass;
+// here no tile waits on another.
 #include <algorithm>
 
 #include "svo_internal.h"
@@ -42,18 +46,8 @@ __host__ __device__ inline uint32_t digit_mask(int bits, int shift) {
   return (1u << w) - 1u;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_sort_clear(uint64_t* status, uint32_t* hist,
-                                                      uint32_t* tickets,
-                                                      const unsigned long long* n_ptr) {
-  const uint64_t tiles = (*n_ptr + TILE - 1) / TILE;
-  const uint64_t words = tiles * RADIX;
-  const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-  for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < words; i += stride)
-    status[i] = 0;
-  if (blockIdx.x == 0) {
-    for (int i = threadIdx.x; i < MAX_PASSES * RADIX; i += BLOCK) hist[i] = 0;
-    if (threadIdx.x < MAX_PASSES) tickets[threadIdx.x] = 0;
-  }
+__global__ __launch_bounds__(BLOCK) void k_sort_clear(uint32_t* hist) {
+  for (int i = threadIdx.x; i < MAX_PASSES * RADIX; i += BLOCK) hist[i] = 0;
 }
 
 // Each thread owns UP_KEYS consecutive keys and adds runs of equal digits to
@@ -113,33 +107,106 @@ __global__ __launch_bounds__(BLOCK) void k_scan_hist(const uint32_t* hist, int p
   }
 }
 
-// One digit pass.  Blocks loop taking tickets; a ticket holder is always a
-// running block, so the look-back of tile t only waits on tiles < t whose
-// blocks are already past their ticket and publish without waiting on any
-// later tile: every wait ends.  Tickets at or past the last tile exit.
+// Digit counts of tiles blockIdx.x, + gridDim.x, ...: counts[d * tiles + t].
 template <typename K>
-__global__ __launch_bounds__(BLOCK) void k_onesweep(const K* in, K* out,
-                                                    const unsigned long long* n_ptr, int shift,
-                                                    uint32_t dm, const uint32_t* offs,
-                                                    uint64_t* status,
-                                                    uint32_t* ticket, uint32_t epoch) {
+__global__ __launch_bounds__(BLOCK) void k_tile_hist(const K* in, const unsigned long long* n_ptr,
+                                                     int shift, uint32_t dm, uint32_t* counts,
+                                                     uint64_t tiles) {
+  __shared__ uint32_t h[WAVES][RADIX];
+  const uint32_t w = threadIdx.x >> 6;
+  const uint64_t n = *n_ptr;
+  const uint64_t n_tiles = (n + TILE - 1) / TILE;
+  const bool vec = ((uintptr_t)in & 15) == 0;
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    for (int i = __lane_id(); i < RADIX; i += 64) h[w][i] = 0;
+    __syncthreads();
+    // thread: KPT consecutive keys of the tile; runs of equal digits are
+    // added at once
+    const uint64_t b = tile * TILE + (uint64_t)threadIdx.x * KPT;
+    const uint32_t cnt = b < n ? (uint32_t)((n - b) < KPT ? (n - b) : KPT) : 0u;
+    K kk[KPT];
+    load_consecutive(in + (b < n ? b : 0), vec && b < n, cnt, kk);
+    if (cnt) {
+      uint32_t last = (uint32_t)(kk[0] >> shift) & dm, run = 1;
+#pragma unroll
+      for (int j = 1; j < KPT; ++j) {
+        if ((uint32_t)j < cnt) {
+          const uint32_t d = (uint32_t)(kk[j] >> shift) & dm;
+          if (d == last) {
+            ++run;
+          } else {
+            atomicAdd(&h[w][last], run);
+            last = d;
+            run = 1;
+          }
+        }
+      }
+      atomicAdd(&h[w][last], run);
+    }
+    __syncthreads();
+    {
+      const uint32_t d = threadIdx.x;
+      uint32_t c = 0;
+#pragma unroll
+      for (int k = 0; k < WAVES; ++k) c += h[k][d];
+      counts[(uint64_t)d * tiles + tile] = c;
+    }
+    __syncthreads();
+  }
+}
+
+// Block d: exclusive scan of counts[d * tiles + 0 .. n_tiles) in place, plus
+// the digit's offset in the pass's output.
+__global__ __launch_bounds__(BLOCK) void k_tile_scan(uint32_t* counts,
+                                                     const unsigned long long* n_ptr,
+                                                     uint64_t tiles, const uint32_t* offs) {
+  __shared__ uint32_t tmp[4];
+  const uint64_t n_tiles = (*n_ptr + TILE - 1) / TILE;
+  uint32_t* c = counts + (uint64_t)blockIdx.x * tiles;
+  uint32_t run = offs[blockIdx.x];
+  constexpr int PER = 16;
+  for (uint64_t t0 = 0; t0 < n_tiles; t0 += (uint64_t)BLOCK * PER) {
+    const uint64_t i0 = t0 + (uint64_t)threadIdx.x * PER;
+    uint32_t v[PER], sum = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      v[j] = i0 + j < n_tiles ? c[i0 + j] : 0u;
+      sum += v[j];
+    }
+    uint32_t total;
+    uint32_t p = run + block_excl_scan256(sum, tmp, total);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      if (i0 + j < n_tiles) c[i0 + j] = p;
+      p += v[j];
+    }
+    run += total;
+    __syncthreads();  // tmp is reused by the next chunk's scan
+  }
+}
+
+// One digit pass over tiles blockIdx.x, + gridDim.x, ...: the tile's keys
+// are ranked stably, reordered in LDS by digit and written to where
+// k_tile_scan put the tile's run of each digit.
+template <typename K>
+__global__ __launch_bounds__(BLOCK) void k_scatter(const K* in, K* out,
+                                                   const unsigned long long* n_ptr, int shift,
+                                                   uint32_t dm, const uint32_t* starts,
+                                                   uint64_t tiles) {
   __shared__ uint32_t wcnt[WAVES][RADIX];
   __shared__ uint32_t dbase[RADIX];
   __shared__ uint32_t tstart[RADIX];
   __shared__ uint32_t tmp[4];
-  __shared__ uint32_t s_tile;
   __shared__ K sorted[TILE];
   const uint32_t w = threadIdx.x >> 6;
   const uint32_t lane = __lane_id();
   const uint64_t n = *n_ptr;
-  const uint32_t d_off = offs[threadIdx.x];
-  while (true) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+  const uint64_t n_tiles = (n + TILE - 1) / TILE;
+  for (uint64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
     for (int i = lane; i < RADIX; i += 64) wcnt[w][i] = 0;
+    dbase[threadIdx.x] = starts[(uint64_t)threadIdx.x * tiles + tile];
     __syncthreads();
-    const uint64_t tile = s_tile;
     const uint64_t tb = tile * TILE;
-    if (tb >= n) return;  // uniform over the block
     const uint32_t tn = (uint32_t)((n - tb) < TILE ? (n - tb) : TILE);
     // loads first, all in flight together (indices clamped, so unconditional)
     K keys[KPT];
@@ -183,8 +250,6 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const K* in, K* out,
         wcnt[k][d] = cnt;
         cnt += c;
       }
-      const uint32_t prefix = lookback_exclusive(status + d, RADIX, tile, cnt, epoch);
-      dbase[d] = d_off + prefix;
       uint32_t total;
       tstart[d] = block_excl_scan256(cnt, tmp, total);
     }
@@ -221,7 +286,7 @@ __global__ __launch_bounds__(BLOCK) void k_onesweep(const K* in, K* out,
 
 size_t radix_sort_tmp_bytes(uint64_t cap) {
   const uint64_t tiles = (std::max<uint64_t>(cap, 1) + TILE - 1) / TILE;
-  return (size_t)tiles * RADIX * sizeof(uint64_t) + 2 * MAX_PASSES * RADIX * sizeof(uint32_t) +
+  return (size_t)tiles * RADIX * sizeof(uint32_t) + 2 * MAX_PASSES * RADIX * sizeof(uint32_t) +
          256;
 }
 
@@ -229,17 +294,14 @@ template <typename K>
 void launch_radix_sort(K* a, K* b, const unsigned long long* n, uint64_t cap, int bits,
                        uint32_t blocks, void* tmp, K** result, hipStream_t s) {
   const uint64_t tiles = (std::max<uint64_t>(cap, 1) + TILE - 1) / TILE;
-  uint64_t* status = (uint64_t*)tmp;
-  uint32_t* hist = (uint32_t*)(status + tiles * RADIX);
+  uint32_t* counts = (uint32_t*)tmp;
+  uint32_t* hist = counts + tiles * RADIX;
   uint32_t* offs = hist + MAX_PASSES * RADIX;
-  uint32_t* tickets = offs + MAX_PASSES * RADIX;
   const int passes = std::min((bits + 7) / 8, MAX_PASSES);
   K* src = a;
   K* dst = b;
   if (passes > 0) {
-    const uint32_t cb = (uint32_t)std::min<uint64_t>(2048, (tiles * RADIX + BLOCK - 1) / BLOCK);
-    hipLaunchKernelGGL(k_sort_clear, dim3(std::max<uint32_t>(cb, 1)), dim3(BLOCK), 0, s, status,
-                       hist, tickets, n);
+    hipLaunchKernelGGL(k_sort_clear, dim3(1), dim3(BLOCK), 0, s, hist);
     const uint64_t chunks = (cap + UP_KEYS - 1) / UP_KEYS;
     const uint32_t ub =
         (uint32_t)std::min<uint64_t>(2048, std::max<uint64_t>(1, (chunks + BLOCK - 1) / BLOCK));
@@ -247,12 +309,17 @@ void launch_radix_sort(K* a, K* b, const unsigned long long* n, uint64_t cap, in
     hipLaunchKernelGGL(k_scan_hist, dim3(1), dim3(BLOCK), 0, s, (const uint32_t*)hist, passes,
                        offs);
   }
+  // tiles are independent: at most `blocks` workgroups, each taking tiles
+  // blockIdx.x, + gridDim.x, ...
   const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, blocks));
   for (int p = 0; p < passes; ++p) {
-    hipLaunchKernelGGL(k_onesweep<K>, dim3(grid), dim3(BLOCK), 0, s, (const K*)src, dst, n,
-                       8 * p, digit_mask(bits, 8 * p), (const uint32_t*)(offs + p * RADIX),
-                       status, tickets + p,
-                       (uint32_t)(p + 1));
+    const uint32_t dm = digit_mask(bits, 8 * p);
+    hipLaunchKernelGGL(k_tile_hist<K>, dim3(grid), dim3(BLOCK), 0, s, (const K*)src, n, 8 * p,
+                       dm, counts, tiles);
+    hipLaunchKernelGGL(k_tile_scan, dim3(RADIX), dim3(BLOCK), 0, s, counts, n, tiles,
+                       (const uint32_t*)(offs + p * RADIX));
+    hipLaunchKernelGGL(k_scatter<K>, dim3(grid), dim3(BLOCK), 0, s, (const K*)src, dst, n,
+                       8 * p, dm, (const uint32_t*)counts, tiles);
     K* t = src;
     src = dst;
     dst = t;

@@ -156,7 +156,7 @@ int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap,
 /* Tuning options of the context (no counterpart in the reference), set
  * between builds:
  *   "small_max_cells"        box cells up to which one lane voxelizes a triangle (512)
- *   "sort_blocks"            persistent grid bound of the radix sort (1024)
+ *   "sort_blocks"            grid bound of the radix sort passes (65536)
  *   "scan_blocks"            persistent grid bound of the level passes (1024)
  *   "bitmap_max_bytes"       largest occupancy bitmap used instead of the sort (32 GiB)
  *   "bitmap_bytes_per_cell"  bitmap bytes allowed per candidate cell (64)
