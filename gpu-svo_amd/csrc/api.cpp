@@ -350,7 +350,7 @@ int svo_ctx_autotune(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params
   Tuning best = base;
   best_ms = default_ms;
   uint32_t trials = 1;
-  const uint32_t smalls[] = {64, 128, 256, 512, 1024};
+  const uint32_t smalls[] = {128, 256, 512, 1024, 2048};
   const uint32_t lanes[] = {1, 2, 3, 4};
   for (uint32_t sm : smalls)
     for (uint32_t ln : lanes) {

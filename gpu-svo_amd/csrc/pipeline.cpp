@@ -568,11 +568,11 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   // Triangles with at most small_max_cells box cells are voxelized one lane
   // each, larger ones by whole waves over 256-row items.  A pass with few
   // triangles has too few lanes to hide a long per-lane loop, so its limit
-  // shrinks with the triangle count (32 cells below 4K triangles, the full
-  // limit from 64K triangles).
+  // shrinks with the triangle count: T / 128 cells (32 below 4K triangles,
+  // 512 at 64K, the default 1024 from 128K).
   Tuning tn = C.tuning;
-  tn.small_max_cells = (uint32_t)std::min<uint64_t>(
-      tn.small_max_cells, std::max<uint64_t>(32, (uint64_t)tn.small_max_cells * T / 65536));
+  tn.small_max_cells = (uint32_t)std::min<uint64_t>(tn.small_max_cells,
+                                                    std::max<uint64_t>(32, T / 128));
   launch_classify(d_md, d_par, d_tb, n, T, tn, d_cnt, s);
   timer.mark("classify");
   HIP_TRY(hipMemcpyAsync(L.pinned, d_cnt, 16, hipMemcpyDeviceToHost, s));

@@ -11,7 +11,7 @@ namespace svo {
 
 // ------------------------------------------------------------ tuning knobs
 struct Tuning {
-  uint32_t small_max_cells = 512;   // bbox cells handled by one lane
+  uint32_t small_max_cells = 1024;  // bbox cells handled by one lane
   uint32_t chunk_cells = 4096;      // cells per big-triangle work item
   uint32_t sort_blocks = 65536;     // grid bound of the radix sort passes (tiles stride)
   uint32_t scan_blocks = 1024;      // fixed grid of the level / attribute scans

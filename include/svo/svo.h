@@ -155,7 +155,7 @@ int svo_ctx_set_timing(svo_ctx* ctx, int enable);
 int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap, int* n);
 /* Tuning options of the context (no counterpart in the reference), set
  * between builds:
- *   "small_max_cells"        box cells up to which one lane voxelizes a triangle (512)
+ *   "small_max_cells"        box cells up to which one lane voxelizes a triangle (1024)
  *   "sort_blocks"            grid bound of the radix sort passes (65536)
  *   "scan_blocks"            persistent grid bound of the level passes (1024)
  *   "bitmap_max_bytes"       largest occupancy bitmap used instead of the sort (32 GiB)
@@ -168,7 +168,7 @@ int svo_ctx_set_option(svo_ctx* ctx, const char* name, int64_t value);
 int svo_ctx_get_option(svo_ctx* ctx, const char* name, int64_t* value);
 /* Tuning search (no counterpart in the reference): builds the sample batch
  * `reps` times (after one warm-up; the median time counts) under the current
- * options and under each combination of small_max_cells in {64, 128, 256, 512, 1024} and
+ * options and under each combination of small_max_cells in {128, 256, 512, 1024, 2048} and
  * batch_lanes in {1, 2, 3, 4}, keeps the fastest on the context and reports
  * it.  The trees built are freed. */
 typedef struct {
