@@ -12,9 +12,9 @@
 //                 lanes walk their box row by row, and the yz-projection
 //                 test, which does not depend on x, is evaluated once per
 //                 row (an exact hoist: the same operands, the same rounding).
-//                 Larger triangles are cut into chunks of 256 rows and queued.
-//  * k_vox_big    one wave64 per queued item of 256 rows: each lane takes
-//                 four rows, prunes each to the x-range outside of which a
+//                 Larger triangles are cut into chunks of 512 rows and queued.
+//  * k_vox_big    one wave64 per queued item of 512 rows: each lane takes
+//                 eight rows, prunes each to the x-range outside of which a
 //                 test provably fails (row_range) and tests the cells left
 //                 in it.
 // Hits are staged per wave in LDS and flushed with one atomic per ~1K keys.
@@ -412,8 +412,9 @@ struct VoxOut {
   unsigned long long* count;
 };
 
-// Big-triangle work item: 256 rows (y, z) of the box, four per lane.
-constexpr uint64_t BIG_ROWS = 256;
+// Big-triangle work item: 512 rows (y, z) of the box, eight per lane (256
+// and 128 rows measured slower: more items, each paying the triangle set-up).
+constexpr uint64_t BIG_ROWS = 512;
 
 __global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
                                                     const MeshParams* params,
@@ -551,7 +552,7 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
   else st.finish();
 }
 
-// One wave per work item of 256 rows: each lane prunes four rows of the item
+// One wave per work item of 512 rows: each lane prunes eight rows of the item
 // to their x-ranges and walks them; each inner loop runs as long as the
 // longest pruned row of the 64.
 template <typename K, bool BITMAP>

@@ -566,7 +566,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   launch_bbox(d_md, n, max_points, d_acc, s);
   launch_params(d_md, n, d_acc, d_par, s);
   // Triangles with at most small_max_cells box cells are voxelized one lane
-  // each, larger ones by whole waves over 256-row items.  A pass with few
+  // each, larger ones by whole waves over 512-row items.  A pass with few
   // triangles has too few lanes to hide a long per-lane loop, so its limit
   // shrinks with the triangle count: T / 128 cells (32 below 4K triangles,
   // 512 at 64K, the default 1024 from 128K).
