@@ -8,7 +8,8 @@
 // Work distribution (the reference gives one thread per triangle):
 //  * k_classify   one lane per triangle: box cells -> key capacity and the
 //                 number of big-triangle chunks (exact, read back once).
-//  * k_vox_small  one lane per triangle whose box has <= small_max cells;
+//  * k_vox_small  one lane per triangle whose box has <= small_max cells,
+//                 the workgroup's triangles handed out in order of box size;
 //                 lanes walk their box row by row, and the yz-projection
 //                 test, which does not depend on x, is evaluated once per
 //                 row (an exact hoist: the same operands, the same rounding).
@@ -478,7 +479,41 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
   BitSink bs;
   bs.bits = out.bits;
   bs.count = out.count;
-  const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  // The workgroup's 256 triangles are handed to its lanes in order of box
+  // size (log2 classes of an estimate of the box cells; estimated big ones
+  // last): a wave's lanes then walk boxes of about one size instead of
+  // waiting for the largest of 64 random ones.  The order of the triangles
+  // does not change the hits.
+  __shared__ uint32_t s_hist[16], s_perm[BLOCK];
+  {
+    const uint64_t own = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t cls = 0;
+    if (own < n_tris) {
+      const uint32_t m = find_mesh(tri_base, n_meshes, own);
+      const MeshDev& md = meshes[m];
+      const MeshParams& P = params[m];
+      float v[3][3];
+      if (P.status == 0 && load_tri(md, P, own - md.tri_base, v)) {
+        float est = 1.0f;
+        for (int a = 0; a < 3; ++a) {
+          const float lo = fminf(v[0][a], fminf(v[1][a], v[2][a]));
+          const float hi = fmaxf(v[0][a], fmaxf(v[1][a], v[2][a]));
+          est *= __fdividef(hi - lo, P.unit[a]) + 1.0f;
+        }
+        const uint32_t ce = !(est >= 1.0f) ? 1u : (est > 1e9f ? 1000000000u : (uint32_t)est);
+        cls = ce > small_max ? 15u : min(14u, 32u - (uint32_t)__builtin_clz(ce));  // 1 .. 14
+      }
+    }
+    if (threadIdx.x < 16) s_hist[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t rank = atomicAdd(&s_hist[cls], 1u);
+    __syncthreads();
+    uint32_t at = rank;
+    for (uint32_t c = 0; c < cls; ++c) at += s_hist[c];
+    s_perm[at] = threadIdx.x;
+    __syncthreads();
+  }
+  const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + s_perm[threadIdx.x];
   TriSetup S;
   uint64_t count = 0;
   K mesh_key = 0;
