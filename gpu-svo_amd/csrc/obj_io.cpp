@@ -61,9 +61,11 @@ void parse_lines(const char* beg, const char* end, ParsedChunk& out) {
       if (std::sscanf(buf, "f %d//%d %d//%d %d//%d", &f0, &g0, &f1, &g1, &f2, &g2) == 6 ||
           std::sscanf(buf, "f %d/%d %d/%d %d/%d", &f0, &g0, &f1, &g1, &f2, &g2) == 6 ||
           std::sscanf(buf, "f %d %d %d", &f0, &f1, &f2) == 3) {
-        out.tris.push_back(f0 - 1);
-        out.tris.push_back(f1 - 1);
-        out.tris.push_back(f2 - 1);
+        // index - 1 with the two's-complement wrap (the reference's int
+        // subtraction overflows for INT_MIN; its compiled code wraps)
+        out.tris.push_back((int32_t)((uint32_t)f0 - 1u));
+        out.tris.push_back((int32_t)((uint32_t)f1 - 1u));
+        out.tris.push_back((int32_t)((uint32_t)f2 - 1u));
       }
     }
   }

@@ -810,9 +810,11 @@ int oracle_load_obj(const char* path, float** points, uint64_t* n_points, int32_
       if (std::sscanf(buffer, "f %d//%d %d//%d %d//%d", &f0, &_f0, &f1, &_f1, &f2, &_f2) == 6 ||
           std::sscanf(buffer, "f %d/%d %d/%d %d/%d", &f0, &_f0, &f1, &_f1, &f2, &_f2) == 6 ||
           std::sscanf(buffer, "f %d %d %d", &f0, &f1, &f2) == 3) {
-        F.push_back(f0 - 1);
-        F.push_back(f1 - 1);
-        F.push_back(f2 - 1);
+        // f - 1 wraps for INT_MIN (undefined in the reference's int
+        // arithmetic; two's complement in its compiled code)
+        F.push_back((int32_t)((uint32_t)f0 - 1u));
+        F.push_back((int32_t)((uint32_t)f1 - 1u));
+        F.push_back((int32_t)((uint32_t)f2 - 1u));
       }
     }
   }

@@ -106,6 +106,63 @@ def test_large_file_parallel_parse(tmp_path):
     assert np.array_equal(P2, P) and np.array_equal(T2, T)
 
 
+def test_extreme_face_indices(tmp_path):
+    # index - 1 wraps for INT_MIN (two's complement); out-of-range values are
+    # kept as parsed and rejected later by validation
+    text = ("v 0 0 0\nv 1 0 0\nv 0 1 0\n"
+            "f -2147483648 2147483647 0\n"
+            "f 1e39 nan -inf\n"          # not integers: %d reads 1, then fails
+            "v 1e39 nan -inf\n")         # overflow to inf and the nan/inf words
+    st, P, T = _both(_write(tmp_path, "extreme.obj", text))
+    assert st == 0
+    assert T.shape == (1, 3) and T[0].tolist() == [2147483647, 2147483646, -1]
+    assert P.shape == (4, 3) and np.isinf(P[3, 0]) and np.isnan(P[3, 1])
+
+
+def _mutate(rng, base, n_ops):
+    s = bytearray(base)
+    for _ in range(n_ops):
+        pos = int(rng.integers(len(s) + 1))
+        op = rng.random()
+        if op < 0.3 and pos < len(s):
+            s[pos] = int(rng.integers(256))
+        elif op < 0.5:
+            del s[pos:pos + int(rng.integers(1, 21))]
+        elif op < 0.75:
+            s[pos:pos] = rng.integers(0, 256, int(rng.integers(1, 300)), dtype=np.uint8).tobytes()
+        elif op < 0.85:
+            s[pos:pos] = b"\n" * int(rng.integers(1, 4))
+        else:
+            s[pos:pos] = b"f 2147483647 -2147483648 99999999999999999999\nv 1e39 nan -inf\n"
+    return bytes(s)
+
+
+def test_fuzzed_files_agree_with_oracle(tmp_path):
+    # random byte edits of a valid file (NULs, CRs, overlong lines, binary
+    # junk, no final newline): status, points and indices must match the
+    # oracle's istream/sscanf reading on every file; two files of several
+    # MiB take the product's multi-threaded split
+    rng = np.random.default_rng(11)
+    lines = ["v %.6g %.6g %.6g\n" % tuple(rng.random(3)) for _ in range(150)]
+    forms = ["f %d %d %d\n", "f %d//1 %d//2 %d//3\n", "f %d/1 %d/2 %d/3\n", "f %d/1/1 %d/2/2 %d/3/3\n"]
+    lines += [forms[int(rng.integers(4))] % tuple(rng.integers(-5, 200, 3)) for _ in range(250)]
+    base = "".join(lines).encode()
+    statuses = set()
+    for k in range(120):
+        data = _mutate(rng, base, int(rng.integers(0, 30)))
+        if rng.random() < 0.3:
+            data = data.rstrip(b"\n")
+        st, _, _ = _both(_write(tmp_path, f"fz{k}.obj", data, binary=True))
+        statuses.add(st)
+    for k in range(2):
+        data = _mutate(rng, base * 400, 40)
+        if k:
+            data += b"x" * 300  # overlong final line without a newline
+        st, _, _ = _both(_write(tmp_path, f"fzbig{k}.obj", data, binary=True))
+        statuses.add(st)
+    assert statuses == {0, -6}  # both outcomes exercised
+
+
 def expected_tree_text(origin, width):
     C = [(0, 1, 1), (0, 0, 1), (0, 1, 0), (0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 0, 1), (1, 1, 1)]
     Lp = [(1, 2), (2, 7), (7, 8), (8, 1), (3, 4), (4, 5), (5, 6), (6, 3), (3, 1), (4, 2), (5, 7),
