@@ -702,15 +702,18 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   timer.mark("levels");
 
   // ---- readback of the level phase
-  const size_t rb = 8 * (NL + 2) + sizeof(LevelSpan) * n * NL + sizeof(MeshTreeState) * n +
-                    sizeof(MeshParams) * n + 64;
-  SVO_TRY(L.ensure_pinned(rb));
+  // counts, spans, states, params, then the three 8-byte hit counters on an
+  // 8-byte boundary (76-byte MeshParams leave an odd count 4-byte aligned)
+  const size_t off_spans = 8 * (NL + 2), off_state = off_spans + sizeof(LevelSpan) * n * NL,
+               off_par = off_state + sizeof(MeshTreeState) * n,
+               off_hits = (off_par + sizeof(MeshParams) * n + 7) & ~(size_t)7;
+  SVO_TRY(L.ensure_pinned(off_hits + 24));
   char* hp = (char*)L.pinned;
   uint64_t* h_counts = (uint64_t*)hp;
-  LevelSpan* h_spans = (LevelSpan*)(hp + 8 * (NL + 2));
-  MeshTreeState* h_state = (MeshTreeState*)((char*)h_spans + sizeof(LevelSpan) * n * NL);
-  MeshParams* h_par = (MeshParams*)((char*)h_state + sizeof(MeshTreeState) * n);
-  uint64_t* h_hits = (uint64_t*)((char*)h_par + sizeof(MeshParams) * n);
+  LevelSpan* h_spans = (LevelSpan*)(hp + off_spans);
+  MeshTreeState* h_state = (MeshTreeState*)(hp + off_state);
+  MeshParams* h_par = (MeshParams*)(hp + off_par);
+  uint64_t* h_hits = (uint64_t*)(hp + off_hits);
   HIP_TRY(hipMemcpyAsync(h_counts, d_counts, 8 * (NL + 2), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(h_spans, d_spans, sizeof(LevelSpan) * n * NL, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(h_state, d_state, sizeof(MeshTreeState) * n, hipMemcpyDeviceToHost, s));
@@ -1017,48 +1020,10 @@ int run_pass(Context& C, const std::vector<MeshInput>& in, const std::vector<uin
   return st;
 }
 
-}  // namespace
-
-int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree*>& trees,
-              std::vector<int>& statuses) {
-  trees.assign(in.size(), nullptr);
-  statuses.assign(in.size(), SVO_OK);
-  std::vector<uint32_t> valid;
-  for (uint32_t i = 0; i < in.size(); ++i) {
-    int kb = 32;
-    const int st = validate_params(&in[i].desc, &in[i].params, &kb);
-    statuses[i] = st;
-    if (st != SVO_OK) continue;
-    if (!valid.empty() && (in[i].params.semantics != in[valid[0]].params.semantics ||
-                           in[i].params.flags != in[valid[0]].params.flags)) {
-      statuses[i] = SVO_E_INVALID;  // one semantics / flag set per pass
-      continue;
-    }
-    valid.push_back(i);
-  }
-  if (valid.empty()) return SVO_OK;
-  CallTiming ct;
-  if (C.timing) {
-    hipSetDevice(C.device);
-    if (hipEventCreate(&ct.ref) == hipSuccess) {
-      // on the null stream, which the lanes' non-blocking streams do not
-      // wait for and nothing else uses: it completes when the call begins
-      if (hipEventRecord(ct.ref, nullptr) != hipSuccess) {
-        hipEventDestroy(ct.ref);
-        ct.ref = nullptr;
-      }
-    } else {
-      ct.ref = nullptr;
-    }
-  }
-  struct RefGuard {
-    CallTiming& ct;
-    Context& C;
-    ~RefGuard() {
-      ct.publish(C);
-      if (ct.ref) hipEventDestroy(ct.ref);
-    }
-  } ref_guard{ct, C};
+// Meshes `valid` (one semantics and flag set) as one pass, or as concurrent
+// sub-batch passes on separate lanes when there are enough of them.
+int run_set(Context& C, const std::vector<MeshInput>& in, const std::vector<uint32_t>& valid,
+            std::vector<svo_tree*>& trees, std::vector<int>& statuses, CallTiming& ct) {
   // Large batches are split into sub-batches that run as concurrent passes on
   // separate lanes (streams): their host round trips and kernel tails
   // overlap.
@@ -1098,6 +1063,58 @@ int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree
   for (int r : rc)
     if (r != SVO_OK) return r;
   return SVO_OK;
+}
+
+}  // namespace
+
+int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree*>& trees,
+              std::vector<int>& statuses) {
+  trees.assign(in.size(), nullptr);
+  statuses.assign(in.size(), SVO_OK);
+  // A pass computes one semantics and one flag set: the valid meshes are
+  // grouped by them, in order of first appearance, one set of passes each.
+  std::vector<std::vector<uint32_t>> sets;
+  for (uint32_t i = 0; i < in.size(); ++i) {
+    int kb = 32;
+    const int st = validate_params(&in[i].desc, &in[i].params, &kb);
+    statuses[i] = st;
+    if (st != SVO_OK) continue;
+    size_t k = 0;
+    while (k < sets.size() && (in[sets[k][0]].params.semantics != in[i].params.semantics ||
+                               in[sets[k][0]].params.flags != in[i].params.flags))
+      ++k;
+    if (k == sets.size()) sets.emplace_back();
+    sets[k].push_back(i);
+  }
+  if (sets.empty()) return SVO_OK;
+  CallTiming ct;
+  if (C.timing) {
+    hipSetDevice(C.device);
+    if (hipEventCreate(&ct.ref) == hipSuccess) {
+      // on the null stream, which the lanes' non-blocking streams do not
+      // wait for and nothing else uses: it completes when the call begins
+      if (hipEventRecord(ct.ref, nullptr) != hipSuccess) {
+        hipEventDestroy(ct.ref);
+        ct.ref = nullptr;
+      }
+    } else {
+      ct.ref = nullptr;
+    }
+  }
+  struct RefGuard {
+    CallTiming& ct;
+    Context& C;
+    ~RefGuard() {
+      ct.publish(C);
+      if (ct.ref) hipEventDestroy(ct.ref);
+    }
+  } ref_guard{ct, C};
+  int rc = SVO_OK;
+  for (const auto& set : sets) {
+    const int r = run_set(C, in, set, trees, statuses, ct);
+    if (rc == SVO_OK) rc = r;
+  }
+  return rc;
 }
 
 }  // namespace svo

@@ -420,13 +420,16 @@ class Context:
                 d, k = _host_desc(pts, tri)
                 keep.append(k)
                 descs[i] = d
-            params[i] = make_params(res, semantics, key_bits, flags)
+            sem = semantics[i] if isinstance(semantics, (list, tuple)) else semantics
+            fl = flags[i] if isinstance(flags, (list, tuple)) else flags
+            params[i] = make_params(res, sem, key_bits, fl)
         return descs, params, keep
 
     def build_batch(self, meshes, semantics=SEM_REFERENCE, key_bits=0, flags=0,
                     on_device=False):
         """meshes: list of (points, tris, res) (host arrays) or, with on_device,
-        (points_ptr, n_points, tris_ptr, n_tris, res).  Returns (trees, statuses)."""
+        (points_ptr, n_points, tris_ptr, n_tris, res); semantics and flags are
+        one value or one per mesh.  Returns (trees, statuses)."""
         n = len(meshes)
         descs, params, keep = self._batch_args(meshes, semantics, key_bits, flags, on_device)
         out = (C.c_void_p * n)()

@@ -217,7 +217,8 @@ int svo_build(svo_ctx* ctx, const svo_mesh_desc* mesh, const svo_params* params,
 /* Mesh arrays already in device memory of the context's device. */
 int svo_build_device(svo_ctx* ctx, const svo_mesh_desc* mesh, const svo_params* params,
                      svo_tree** out);
-/* Many meshes in one pipeline pass (composite (mesh, Morton) keys).
+/* Many meshes in one pipeline pass (composite (mesh, Morton) keys); meshes
+ * with different semantics or flags run as one pass per (semantics, flags).
  * meshes_on_device: 0 = host arrays, 1 = device arrays.  statuses may be NULL.
  * Returns SVO_OK if every mesh built; per-mesh results in statuses[i]. */
 int svo_build_batch(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params* params,

@@ -84,6 +84,32 @@ def test_batch_mixed_status(gpu_ctx):
             t.free()
 
 
+def test_batch_mixed_semantics_and_flags(gpu_ctx):
+    """One call with both semantics and both flag sets: every mesh builds (one
+    set of passes per (semantics, flags), the larger sets split over lanes)
+    and equals its plain build."""
+    import svo_amd
+    meshes = _meshes(80, seed=23)
+    n = len(meshes)
+    assert n >= 50
+    # 36+ meshes of (reference, no flags): enough for two concurrent passes
+    sems = [0 if i < 36 else i % 2 for i in range(n)]
+    flags = [svo_amd.BUILD_NO_VERTEX_EDGE if i >= 36 and i % 3 == 2 else 0 for i in range(n)]
+    print(f"mixed batch of {n} meshes", flush=True)
+    trees, st = gpu_ctx.build_batch(meshes, semantics=sems, flags=flags)
+    assert all(s == 0 for s in st), st
+    for i, ((P, T, res), tb) in enumerate(zip(meshes, trees)):
+        tp = gpu_ctx.build(P, T, res, semantics=sems[i], flags=flags[i])
+        assert tb.info.semantics == sems[i]
+        fields = ALL if flags[i] == 0 else [f for f in ALL if f not in ("vertices", "edges", "faces")]
+        for f in fields:
+            assert_same(f"mesh {i} {f}", tb.field(f), tp.field(f))
+        if flags[i]:
+            assert tb.info.n_vertices == 0 and tb.info.n_edges == 0
+        tp.free()
+        tb.free()
+
+
 @pytest.mark.parametrize("path", [1, 2], ids=["bitmap", "sort"])
 def test_batch_odd_resolutions(gpu_ctx, path):
     """Resolutions whose bitmap regions are no whole number of 16-byte pieces

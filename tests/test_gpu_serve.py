@@ -38,15 +38,17 @@ def _drain(srv, n, timeout_s=120):
     return done
 
 
-@pytest.mark.parametrize("sem", [0, 1])
+@pytest.mark.parametrize("sem", [0, 1, "mixed"])
 def test_served_equals_alone(gpu_ctx, sem):
+    # "mixed": requests alternate the two semantics, so batches hold both
     import svo_amd
     reqs = _requests(48, seed=31)
+    sems = [(i % 2 if sem == "mixed" else sem) for i in range(len(reqs))]
     srv = svo_amd.Server(gpu_ctx, mem_budget=32 << 30, max_batch=8, n_lanes=2, max_wait_us=300)
     try:
         ids = []
         for i, (P, T, res) in enumerate(reqs):
-            ids.append(srv.submit(P, T, res, semantics=sem))
+            ids.append(srv.submit(P, T, res, semantics=sems[i]))
             if i % 7 == 0:
                 time.sleep(0.002)  # arrivals spread over time
         done = _drain(srv, len(ids))
@@ -55,10 +57,11 @@ def test_served_equals_alone(gpu_ctx, sem):
     sizes = [done[i].batch_size for i in ids]
     print(f"batch sizes {sorted(set(sizes))}", flush=True)
     assert max(sizes) > 1  # requests were batched together
-    for rid, (P, T, res) in zip(ids, reqs):
+    for rid, (P, T, res), sm in zip(ids, reqs, sems):
         d = done[rid]
         assert d.status == 0
-        alone = gpu_ctx.build(P, T, res, semantics=sem)
+        assert d.tree.info.semantics == sm
+        alone = gpu_ctx.build(P, T, res, semantics=sm)
         for f in FIELDS:
             assert_same(f"request {rid} {f}", d.tree.field(f), alone.field(f))
         # neighbor queries on the resident tree
