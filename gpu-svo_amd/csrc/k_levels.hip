@@ -263,8 +263,11 @@ __device__ inline void clear_words(uint32_t* bits, uint64_t n_words, uint64_t w0
   }
 }
 
-constexpr int SC_PER = 16;                  // tile counts per thread in the scan
-constexpr int SC_CHUNK = BLOCK * SC_PER;     // 4096 tiles per scan workgroup
+// Tile counts per thread in the scan: short chunks spread the loads over
+// more workgroups (4 measured faster than 16, and 64, one workgroup for a
+// res-1024 bitmap, slower still: its loads are latency-bound).
+constexpr int SC_PER = 4;
+constexpr int SC_CHUNK = BLOCK * SC_PER;     // 1024 tiles per scan workgroup
 
 // Per tile: set bits, nonzero bytes, and the 4-byte units k_bm_emit will
 // zero.
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(BLOCK) void k_bm_count(const uint32_t* bits, uint64
   }
 }
 
-// Exclusive scan of the tile counts in place, 4096 tiles per workgroup, chunk
+// Exclusive scan of the tile counts in place, SC_CHUNK tiles per workgroup, chunk
 // offsets by decoupled look-back (chunks claimed by ticket); totals to
 // counts[0] (unique keys = level-1 elements) and counts[1] (groups = level-2
 // elements); the bytes k_bm_emit will zero are added to *reset_bytes.
