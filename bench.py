@@ -87,6 +87,22 @@ def dedup_gbs(stats, stages):
     return round(sum(stage_bytes(stats, k) for k in names) / (t * 1e-3) / 1e9, 1) if t else None
 
 
+def compulsory_bytes(stats):
+    """End-to-end compulsory bytes of a step: the triangles read once and
+    every output written once (the unique voxel codes, the node arrays code,
+    is_leaf, origin, width, parent, childs and neighbors, and the vertex,
+    edge and face records), with nothing re-read."""
+    kb, N = stats["key_bytes"], stats["n_nodes"]
+    return (36 * stats["n_tris"] + kb * stats["n_voxels"] + (kb + 161) * N +
+            16 * stats["n_vertices"] + 28 * stats["n_edges"] + 28 * stats["n_faces"])
+
+
+def compulsory_gbs(stats, ms_per_step):
+    """compulsory_bytes over the whole step's wall time (GB/s): extra passes
+    inside the step can only lower it."""
+    return round(compulsory_bytes(stats) / (ms_per_step * 1e-3) / 1e9, 1)
+
+
 def pass_stats(ctx):
     import ctypes as C
     import svo_amd
@@ -244,6 +260,7 @@ def run_batch_workload(args):
         "options": {o.split("=", 1)[0]: int(o.split("=", 1)[1]) for o in args.option} or None,
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "dedup_gbs": dedup_gbs(stats, stages),
+        "compulsory_gbs": compulsory_gbs(stats, ms_batch),
         "dedup_path": "bitmap" if stats["bitmap_bytes"] else "sort",
         "roofline": _roofline(stats, stages),
     }
@@ -314,6 +331,7 @@ def run_single_workload(args, name, P, T, res, key_bits, flags=0, desc=""):
                   "box_cells": stats["box_cells"], "out_gb": round(stats["out_bytes"] / 1e9, 3)},
         "stages_ms": {k: round(v, 4) for k, v in stages.items()},
         "dedup_gbs": dedup_gbs(stats, stages),
+        "compulsory_gbs": compulsory_gbs(stats, ms),
         "options": {o.split("=", 1)[0]: int(o.split("=", 1)[1]) for o in args.option} or None,
         "dedup_path": "bitmap" if stats["bitmap_bytes"] else "sort",
         "roofline": _roofline(stats, stages),

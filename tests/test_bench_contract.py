@@ -53,6 +53,16 @@ def test_dedup_rate_uses_the_stages_present():
     assert bench.dedup_gbs(s, {"voxelize": 1.0}) is None
 
 
+def test_compulsory_bytes():
+    # 36 B per triangle, 4 B per unique voxel, 165 B of node arrays per node
+    # (u32 code), 16 B per vertex and 28 B per edge record
+    s = _stats()
+    assert bench.compulsory_bytes(s) == 36 * 50 + 4 * 600 + 165 * 900 + 16 * 7200 + 28 * 10800
+    assert bench.compulsory_gbs(s, 2.0) == round(bench.compulsory_bytes(s) / 2e-3 / 1e9, 1)
+    s8 = _stats(key_bytes=8, n_faces=300)
+    assert bench.compulsory_bytes(s8) - bench.compulsory_bytes(s) == 4 * 600 + 4 * 900 + 28 * 300
+
+
 def test_roofline_picks_the_longest_device_stage():
     s = _stats()
     stages = {"setup_b": 9.0, "voxelize": 0.5, "attr_write": 2.0, "nodes": 0.4}
