@@ -115,7 +115,7 @@ typedef struct {
 typedef struct {
   int32_t res[3];     /* surfaceVoxelGridSize, SVO.h:43 */
   uint32_t semantics; /* SVO_SEM_* */
-  uint32_t key_bits;  /* 32 (reference, res <= 1024), 64 (res <= 2^21), 0 = pick */
+  uint32_t key_bits;  /* 32 (reference, res <= 1024), 64 (res <= 65536), 0 = pick */
   uint32_t flags;     /* SVO_BUILD_* */
 } svo_params;
 
