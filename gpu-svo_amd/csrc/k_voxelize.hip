@@ -28,6 +28,7 @@ constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / 64;
 constexpr int STAGE = 1024;        // keys staged per wave
 constexpr int BBOX_CHUNK = 4096;   // points per bbox block
+constexpr uint32_t CLASSIFY_BLOCKS = 1024;  // grid bound of k_classify
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -423,9 +424,11 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
                                                     uint64_t n_tris, uint32_t small_max,
                                                     uint32_t chunk_cells,
                                                     unsigned long long* totals) {
-  const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+  // grid-stride over the triangles with a bounded grid: the two totals take
+  // one atomic per workgroup, and atomics on one address serialize
   uint64_t cells = 0, chunks = 0;
-  if (gid < n_tris) {
+  for (uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; gid < n_tris;
+       gid += (uint64_t)gridDim.x * BLOCK) {
     const uint32_t m = find_mesh(tri_base, n_meshes, gid);
     const MeshDev& md = meshes[m];
     const MeshParams& P = params[m];
@@ -434,8 +437,9 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
       int g0[3], g1[3];
       tri_box(v, P, g0, g1);
       const uint64_t nx = g1[0] - g0[0] + 1, ny = g1[1] - g0[1] + 1, nz = g1[2] - g0[2] + 1;
-      cells = nx * ny * nz;
-      if (cells > small_max) chunks = (ny * nz + BIG_ROWS - 1) / BIG_ROWS;
+      const uint64_t c = nx * ny * nz;
+      cells += c;
+      if (c > small_max) chunks += (ny * nz + BIG_ROWS - 1) / BIG_ROWS;
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -672,8 +676,9 @@ void launch_classify(const MeshDev* meshes, const MeshParams* params, const uint
   const uint32_t blocks = (uint32_t)((n_tris + BLOCK - 1) / BLOCK);
   hipLaunchKernelGGL(k_flag_bad_index, dim3(blocks), dim3(BLOCK), 0, s, meshes,
                      const_cast<MeshParams*>(params), tri_base, n_meshes, n_tris);
-  hipLaunchKernelGGL(k_classify, dim3(blocks), dim3(BLOCK), 0, s, meshes, params, tri_base,
-                     n_meshes, n_tris, tn.small_max_cells, tn.chunk_cells, totals);
+  hipLaunchKernelGGL(k_classify, dim3(std::min<uint32_t>(blocks, CLASSIFY_BLOCKS)), dim3(BLOCK), 0,
+                     s, meshes, params, tri_base, n_meshes, n_tris, tn.small_max_cells,
+                     tn.chunk_cells, totals);
 }
 
 template <typename K>
