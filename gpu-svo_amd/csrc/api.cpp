@@ -89,7 +89,8 @@ int svo_ctx_create(int device, uint64_t mem_budget, svo_ctx** out) {
   if (!out) return SVO_E_INVALID;
   *out = nullptr;
   svo_ctx* c = new svo_ctx();
-  const int st = c->impl.init(device, mem_budget);
+  c->impl = std::make_shared<Context>();
+  const int st = c->impl->init(device, mem_budget);
   if (st != SVO_OK) {
     delete c;
     return st;
@@ -100,14 +101,16 @@ int svo_ctx_create(int device, uint64_t mem_budget, svo_ctx** out) {
 
 int svo_ctx_destroy(svo_ctx* ctx) {
   if (!ctx) return SVO_E_INVALID;
-  ctx->impl.destroy();
-  delete ctx;
+  // servers drive the lanes from their own threads: destroy them first
+  if (ctx->impl->servers.load() > 0) return SVO_E_BUSY;
+  ctx->impl->release_lanes();
+  delete ctx;  // trees still alive keep the rest of the device state
   return SVO_OK;
 }
 
 int svo_ctx_set_timing(svo_ctx* ctx, int enable) {
   if (!ctx) return SVO_E_INVALID;
-  ctx->impl.timing = enable != 0;
+  ctx->impl->timing = enable != 0;
   return SVO_OK;
 }
 
@@ -131,7 +134,7 @@ const OptionRef kOptions[] = {
 
 int svo_ctx_set_option(svo_ctx* ctx, const char* name, int64_t value) {
   if (!ctx || !name) return SVO_E_INVALID;
-  Context& C = ctx->impl;
+  Context& C = *ctx->impl;
   std::lock_guard<std::mutex> lk(C.mu);
   Tuning& t = C.tuning;
   for (const OptionRef& o : kOptions) {
@@ -153,7 +156,7 @@ int svo_ctx_set_option(svo_ctx* ctx, const char* name, int64_t value) {
 
 int svo_ctx_get_option(svo_ctx* ctx, const char* name, int64_t* value) {
   if (!ctx || !name || !value) return SVO_E_INVALID;
-  Context& C = ctx->impl;
+  Context& C = *ctx->impl;
   std::lock_guard<std::mutex> lk(C.mu);
   const Tuning& t = C.tuning;
   const std::string n(name);
@@ -171,7 +174,7 @@ int svo_ctx_get_option(svo_ctx* ctx, const char* name, int64_t* value) {
 
 int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap, int* n) {
   if (!ctx || !n) return SVO_E_INVALID;
-  Context& C = ctx->impl;
+  Context& C = *ctx->impl;
   std::lock_guard<std::mutex> lk(C.stage_mu);
   const int k = (int)C.stage_ms.size();
   *n = k;
@@ -182,30 +185,30 @@ int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap,
   return k > cap ? SVO_E_RANGE : SVO_OK;
 }
 
-uint64_t svo_ctx_bytes_in_use(const svo_ctx* ctx) { return ctx ? ctx->impl.in_use.load() : 0; }
+uint64_t svo_ctx_bytes_in_use(const svo_ctx* ctx) { return ctx ? ctx->impl->in_use.load() : 0; }
 
 int svo_ctx_last_pass_stats(svo_ctx* ctx, svo_pass_stats* out) {
   if (!ctx || !out) return SVO_E_INVALID;
-  std::lock_guard<std::mutex> lk(ctx->impl.stage_mu);
-  *out = ctx->impl.last_pass;
+  std::lock_guard<std::mutex> lk(ctx->impl->stage_mu);
+  *out = ctx->impl->last_pass;
   return SVO_OK;
 }
 
 int svo_device_alloc(svo_ctx* ctx, uint64_t bytes, void** out) {
   if (!ctx || !out) return SVO_E_INVALID;
-  return ctx->impl.dev_malloc(out, bytes);
+  return ctx->impl->dev_malloc(out, bytes);
 }
 
 int svo_device_free(svo_ctx* ctx, void* ptr, uint64_t bytes) {
   if (!ctx) return SVO_E_INVALID;
-  ctx->impl.dev_free(ptr, bytes);
+  ctx->impl->dev_free(ptr, bytes);
   return SVO_OK;
 }
 
 int svo_device_copy(svo_ctx* ctx, void* dst, const void* src, uint64_t bytes, int kind) {
   if (!ctx || kind < 0 || kind > 2) return SVO_E_INVALID;
   if (!bytes) return SVO_OK;
-  Context& C = ctx->impl;
+  Context& C = *ctx->impl;
   const hipMemcpyKind k = kind == 0   ? hipMemcpyHostToDevice
                           : kind == 1 ? hipMemcpyDeviceToHost
                                       : hipMemcpyDeviceToDevice;
@@ -257,7 +260,7 @@ static int build_common(svo_ctx* ctx, const svo_mesh_desc* mesh, const svo_param
   in[0].on_device = on_device != 0;
   std::vector<svo_tree*> trees;
   std::vector<int> st;
-  const int s = run_batch(ctx->impl, in, trees, st);
+  const int s = run_batch(*ctx->impl, in, trees, st);
   if (s != SVO_OK) return s;
   if (st[0] != SVO_OK) return st[0];
   *out = trees[0];
@@ -284,7 +287,7 @@ int svo_build_batch(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params*
   }
   std::vector<svo_tree*> tr;
   std::vector<int> st;
-  const int s = run_batch(ctx->impl, in, tr, st);
+  const int s = run_batch(*ctx->impl, in, tr, st);
   // a failed pass of a split batch marks its own meshes; the others' trees
   // are returned
   int agg = s;
@@ -309,7 +312,7 @@ int svo_ctx_autotune(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params
                      uint32_t n, int meshes_on_device, uint32_t reps, svo_tune_result* out) {
   if (!ctx || !n || !meshes || !params) return SVO_E_INVALID;
   if (reps == 0) reps = 3;
-  Context& C = ctx->impl;
+  Context& C = *ctx->impl;
   Tuning base;
   {
     std::lock_guard<std::mutex> lk(C.mu);

@@ -56,12 +56,13 @@ int Context::init(int dev, uint64_t bud) {
   return SVO_OK;
 }
 
-void Context::destroy() {
+void Context::release_lanes() {
+  std::lock_guard<std::mutex> lk(mu);
   hipSetDevice(device);
   for (auto& l : lanes) {
     if (l->stream) hipStreamSynchronize(l->stream);
-    l->ws.release_all();
     if (l->side) hipStreamSynchronize(l->side);
+    l->ws.release_all();
     for (auto e : l->events) hipEventDestroy(e);
     if (l->fork) hipEventDestroy(l->fork);
     if (l->join) hipEventDestroy(l->join);
@@ -72,6 +73,11 @@ void Context::destroy() {
   lanes.clear();
   free_lanes.clear();
   trim_cache(0);
+}
+
+Context::~Context() {
+  release_lanes();
+  hipSetDevice(device);
   if (aux_stream) hipStreamDestroy(aux_stream);
   aux_stream = nullptr;
 }
@@ -462,6 +468,7 @@ bool no_adjacent_duplicates(const std::vector<MeshNodeLayout>& lay, const MeshPa
 
 struct Plan {
   CallTiming* timing = nullptr;
+  Tuning tuning;              // snapshot taken under the context's mutex
   std::vector<uint32_t> idx;  // active mesh -> input index
   int bits = 1;               // per-axis coordinate bits of the batch
   int cb = 3;                 // code bits
@@ -570,7 +577,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   // triangles has too few lanes to hide a long per-lane loop, so its limit
   // shrinks with the triangle count: T / 128 cells (32 below 4K triangles,
   // 512 at 64K, the default 1024 from 128K).
-  Tuning tn = C.tuning;
+  Tuning tn = plan.tuning;
   tn.small_max_cells = (uint32_t)std::min<uint64_t>(tn.small_max_cells,
                                                     std::max<uint64_t>(32, T / 128));
   launch_classify(d_md, d_par, d_tb, n, T, tn, d_cnt, s);
@@ -585,10 +592,10 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   // ---- de-duplication path: occupancy bitmap (level 1 read straight from
   // it) while it is small enough, else a radix sort of the raw hits
   const uint64_t bm_bytes = 4 * bm_words;
-  bool use_bm = bm_bytes <= C.tuning.bitmap_max_bytes &&
-                bm_bytes <= (uint64_t)C.tuning.bitmap_bytes_per_cell * key_cap;
-  if (C.tuning.force_path == 1) use_bm = true;
-  if (C.tuning.force_path == 2) use_bm = false;
+  bool use_bm = bm_bytes <= plan.tuning.bitmap_max_bytes &&
+                bm_bytes <= (uint64_t)plan.tuning.bitmap_bytes_per_cell * key_cap;
+  if (plan.tuning.force_path == 1) use_bm = true;
+  if (plan.tuning.force_path == 2) use_bm = false;
   // level-1 capacity: unique voxels <= min(candidate cells, bitmap bits)
   const uint64_t u_cap = use_bm ? std::max<uint64_t>(1, std::min<uint64_t>(key_cap, 32 * bm_words))
                                 : key_cap;
@@ -689,16 +696,16 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   } else {
     K* sorted = nullptr;
     launch_radix_sort<K>(keys_a, keys_b, d_cnt + 2, key_cap, plan.cb + plan.mb,
-                         C.tuning.sort_blocks, sort_tmp, &sorted, s);
+                         plan.tuning.sort_blocks, sort_tmp, &sorted, s);
     timer.mark("sort");
     launch_level_first<K>(sorted, d_cnt + 2, key_cap, kl, lv[1], lv[2], d_counts, d_spans, NL,
-                          C.tuning.scan_blocks, lvl_tmp, s);
+                          plan.tuning.scan_blocks, lvl_tmp, s);
     launch_level_first_finalize(d_spans, n, NL, d_state, s);
   }
   timer.mark("unique_level1");
   for (uint32_t l = 2; l <= NL; ++l)
     launch_level<K>((int)l, kl, lv[l], lv[l + 1], d_counts, d_spans, NL, d_state,
-                    C.tuning.scan_blocks, lvl_tmp, s);
+                    plan.tuning.scan_blocks, lvl_tmp, s);
   timer.mark("levels");
 
   // ---- readback of the level phase
@@ -790,10 +797,24 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   {
     void* p = nullptr;
     SVO_TRY(C.dev_malloc(&p, off));
-    out->block->ctx = &C;
+    out->block->ctx = C.shared_from_this();
     out->block->ptr = p;
     out->block->bytes = off;
   }
+  // From here on kernels of this pass write the output block.  On an error
+  // return, wait for both streams before `out` (declared above, so destroyed
+  // after this guard) gives the block back to the context's cache, where
+  // another lane could take it while this pass's kernels still write it.
+  struct DrainOnError {
+    Lane& L;
+    bool armed = true;
+    ~DrainOnError() {
+      if (!armed) return;
+      hipStreamSynchronize(L.stream);
+      hipStreamSynchronize(L.side);
+      hipGetLastError();
+    }
+  } drain{L};
   char* ob = (char*)out->block->ptr;
   NodeArrays& na = out->nodes;
   na.code = ob + o_code;
@@ -876,8 +897,8 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
       timer.mark("attr_count");
       launch_ref_attrs(na, N, d_nbase, n, ao, s);
     } else {
-      SVO_TRY(W.get("attr_tmp", attr_tmp_bytes(C.tuning.scan_blocks), &attr_tmp));
-      const uint32_t ab = grid_for(N, 2048, C.tuning.scan_blocks);
+      SVO_TRY(W.get("attr_tmp", attr_tmp_bytes(plan.tuning.scan_blocks), &attr_tmp));
+      const uint32_t ab = grid_for(N, 2048, plan.tuning.scan_blocks);
       launch_attrs_count(sem, na, N, d_nbase, n, ab, attr_tmp, ao, s);
       timer.mark("attr_count");
       launch_attrs_write(sem, na, N, d_nbase, n, ab, attr_tmp, ao, s);
@@ -920,7 +941,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     if (mstat[i] != SVO_OK) continue;
     svo_tree* t = new svo_tree();
     t->out = out;
-    t->ctx = &C;
+    t->ctx = C.shared_from_this();
     t->node_base = node_base[i];
     t->semantics = sem;
     t->key_bits_out = in[src].params.key_bits ? (int)in[src].params.key_bits
@@ -967,6 +988,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     I.device_bytes = out->block->bytes;
     trees[src] = t;
   }
+  drain.armed = false;  // both streams were synchronized above
   return SVO_OK;
 }
 
@@ -1002,13 +1024,19 @@ int run_pass(Context& C, const std::vector<MeshInput>& in, const std::vector<uin
   }
   plan.k64 = plan.cb + plan.mb > 32;
   plan.code_bytes = any64 ? 8 : 4;
+  {
+    std::lock_guard<std::mutex> lk(C.mu);
+    plan.tuning = C.tuning;
+  }
   Lane* L = C.acquire_lane();
   if (!L) return SVO_E_HIP;
   hipSetDevice(C.device);
   int st = plan.k64 ? run_typed<uint64_t>(C, *L, in, plan, trees, statuses)
                     : run_typed<uint32_t>(C, *L, in, plan, trees, statuses);
   if (st != SVO_OK) {
+    // the lane's workspaces are reused by its next pass: let this one end
     hipStreamSynchronize(L->stream);
+    hipStreamSynchronize(L->side);
     hipGetLastError();
     for (uint32_t i : plan.idx) {
       delete trees[i];

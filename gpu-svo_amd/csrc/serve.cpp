@@ -8,10 +8,12 @@
 // everything already resident or in flight.  Batches built by different
 // workers run concurrently on the device.  Finished trees stay resident until
 // the caller frees them; the budget check sees that through the context's
-// byte counter.  A batch that still runs out of memory goes back to the head
-// of the queue and is retried while its oldest request is less than a second
-// old (other batches or freed trees may release memory meanwhile); after that
-// its requests complete with SVO_E_OOM.
+// byte counter.  Results are per request: a batch can run as several passes,
+// and when one of them fails the requests of the others still complete with
+// their trees.  A request whose pass ran out of memory goes back to the head
+// of the queue and is retried while it is less than a second old (other
+// batches or freed trees may release memory meanwhile); after that it
+// completes with SVO_E_OOM.
 #include <chrono>
 #include <deque>
 #include <thread>
@@ -39,7 +41,7 @@ struct Request {
 }  // namespace
 
 struct svo_server {
-  Context* ctx = nullptr;
+  std::shared_ptr<Context> ctx;
   svo_server_cfg cfg{};
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -50,6 +52,8 @@ struct svo_server {
   uint64_t next_id = 1;
   uint64_t in_flight_bytes = 0;  // estimates of batches being built
   uint32_t in_flight = 0;
+  uint64_t batches_run = 0;      // passes' batches finished
+  uint64_t retried = 0;          // requests re-queued after running out of memory
 
   void worker_loop();
 };
@@ -107,30 +111,44 @@ void svo_server::worker_loop() {
     }
     std::vector<svo_tree*> trees;
     std::vector<int> st;
-    const int s = run_batch(*ctx, in, trees, st);
+    // per-request results: a batch split into several passes can fail in
+    // one pass while the others build their trees
+    run_batch(*ctx, in, trees, st);
     const double t_done = now_s();
     lk.lock();
     in_flight_bytes -= est;
     --in_flight;
-    // out of memory: retry for up to a second while other work or resident
-    // trees may release memory, then report SVO_E_OOM
-    if (s == SVO_E_OOM && !stop && now_s() - batch.front().t_submit < 1.0) {
-      for (auto it = batch.rbegin(); it != batch.rend(); ++it) pending.push_front(std::move(*it));
-      lk.unlock();
-      std::this_thread::sleep_for(std::chrono::microseconds(200));
-      lk.lock();
-      continue;
-    }
+    // requests whose pass ran out of memory go back to the head of the
+    // queue, in arrival order, while they are less than a second old (other
+    // batches or freed trees may release memory meanwhile); the others
+    // complete now
+    std::vector<Request> retry;
     for (size_t i = 0; i < batch.size(); ++i) {
+      if (st[i] == SVO_E_OOM && !stop && now_s() - batch[i].t_submit < 1.0) {
+        if (trees[i]) svo_tree_free(trees[i]);
+        retry.push_back(std::move(batch[i]));
+        continue;
+      }
       svo_completion c{};
       c.request_id = batch[i].id;
-      c.status = s != SVO_OK ? s : st[i];
+      c.status = st[i];
       c.batch_size = (uint32_t)batch.size();
-      c.tree = s == SVO_OK ? trees[i] : nullptr;
+      c.tree = st[i] == SVO_OK ? trees[i] : nullptr;
+      if (!c.tree && trees[i]) svo_tree_free(trees[i]);
       c.t_submit_s = batch[i].t_submit;
       c.t_start_s = t_start;
       c.t_done_s = t_done;
       done.push_back(c);
+    }
+    ++batches_run;
+    if (!retry.empty()) {
+      retried += retry.size();
+      for (auto it = retry.rbegin(); it != retry.rend(); ++it) pending.push_front(std::move(*it));
+      if (retry.size() == batch.size()) {
+        lk.unlock();
+        std::this_thread::sleep_for(std::chrono::microseconds(200));
+        lk.lock();
+      }
     }
     cv_done.notify_all();
     cv_work.notify_all();
@@ -142,7 +160,8 @@ extern "C" {
 int svo_server_create(svo_ctx* ctx, const svo_server_cfg* cfg, svo_server** out) {
   if (!ctx || !cfg || !out) return SVO_E_INVALID;
   svo_server* s = new svo_server();
-  s->ctx = &ctx->impl;
+  s->ctx = ctx->impl;
+  s->ctx->servers.fetch_add(1);
   s->cfg = *cfg;
   if (!s->cfg.max_batch) s->cfg.max_batch = 64;
   if (!s->cfg.max_queue) s->cfg.max_queue = 1 << 16;
@@ -211,6 +230,7 @@ int svo_server_destroy(svo_server* srv) {
   for (auto& t : srv->workers) t.join();
   for (auto& c : srv->done)
     if (c.tree) svo_tree_free(c.tree);
+  srv->ctx->servers.fetch_sub(1);
   delete srv;
   return SVO_OK;
 }

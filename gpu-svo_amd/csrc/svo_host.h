@@ -48,9 +48,10 @@ struct Lane {
   int ensure_pinned(size_t bytes);
 };
 
-// Refcounted device allocation holding a batch's output arrays.
+// Refcounted device allocation holding a batch's output arrays.  It keeps
+// its context alive: a tree may outlive svo_ctx_destroy.
 struct DeviceBlock {
-  Context* ctx = nullptr;
+  std::shared_ptr<Context> ctx;
   void* ptr = nullptr;
   size_t bytes = 0;
   ~DeviceBlock();
@@ -66,27 +67,35 @@ struct BatchOutput {
   int code_bytes = 4;
 };
 
-struct Context {
+// The device state behind an svo_ctx.  Shared: the caller's handle, every
+// tree (through its output block) and every server hold a reference, so
+// trees stay readable after svo_ctx_destroy, which only gives the lanes
+// (streams and workspaces) back; the rest goes with the last reference.
+struct Context : std::enable_shared_from_this<Context> {
   int device = 0;
   uint64_t budget = 0;  // 0 = unbounded
   std::atomic<uint64_t> in_use{0};
   std::atomic<uint64_t> ws_bytes{0};  // part of in_use held by lane workspaces (reused)
   std::atomic<int> active_passes{0};  // pipeline passes running on the context
-  Tuning tuning;
+  std::atomic<int> servers{0};        // live svo_server objects on the context
+  Tuning tuning;                      // guarded by mu; a pass works on a snapshot
   std::mutex mu;
   std::condition_variable cv;
   std::vector<std::unique_ptr<Lane>> lanes;
   std::vector<Lane*> free_lanes;
   hipStream_t aux_stream = nullptr;  // copies for read-out
   std::mutex aux_mu;
-  bool timing = false;
+  std::atomic<bool> timing{false};
   std::mutex stage_mu;
   std::vector<std::string> stage_names;
   std::vector<double> stage_ms;
   svo_pass_stats last_pass{};
 
   int init(int dev, uint64_t budget);
-  void destroy();
+  // Gives the lanes back (synchronizes and destroys their streams, frees
+  // their workspaces) and trims the block cache; idempotent.
+  void release_lanes();
+  ~Context();
   Lane* acquire_lane();
   void release_lane(Lane* l);
   int reserve(uint64_t bytes);  // budget accounting; SVO_E_OOM if over
@@ -106,7 +115,7 @@ struct Context {
 }  // namespace svo
 
 struct svo_ctx {
-  svo::Context impl;
+  std::shared_ptr<svo::Context> impl;
 };
 
 struct svo_mesh {
@@ -116,7 +125,7 @@ struct svo_mesh {
 
 struct svo_tree {
   std::shared_ptr<svo::BatchOutput> out;
-  svo::Context* ctx = nullptr;
+  std::shared_ptr<svo::Context> ctx;
   svo_tree_info info{};
   uint64_t node_base = 0;
   uint64_t vox_base = 0;

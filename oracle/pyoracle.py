@@ -78,6 +78,10 @@ def lib():
                                       C.POINTER(C.c_int32)]
         L.oracle_voxelize.restype = C.c_uint64
         L.oracle_set_arith_variant.argtypes = [C.c_int32]
+        L.oracle_voxelize_alloc.argtypes = [P, C.c_uint64, P, C.c_uint64, P, C.c_int32,
+                                            C.c_int32, C.POINTER(C.c_uint64),
+                                            C.POINTER(C.c_int32)]
+        L.oracle_voxelize_alloc.restype = C.POINTER(C.c_uint64)
         _lib = L
     return _lib
 
@@ -190,8 +194,25 @@ def write_obj(points, tris, res, path, semantics=SEM_REFERENCE, key_bits=32, vox
         L.oracle_tree_free(h)
 
 
-def voxelize(points, tris, res, key_bits=32):
+def voxelize(points, tris, res, key_bits=32, threads=None):
+    """Sorted unique level-1 codes (uint64); threads=None runs the
+    single-threaded count-then-copy form, an int the OpenMP form."""
     L = lib()
+    if threads is not None:
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        tri = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
+        r = _as_res(res)
+        st = C.c_int32()
+        n = C.c_uint64()
+        p = L.oracle_voxelize_alloc(_ptr(pts), pts.shape[0], _ptr(tri), tri.shape[0], _ptr(r),
+                                    key_bits, int(threads), C.byref(n), C.byref(st))
+        if not p:
+            return st.value, np.zeros(0, np.uint64)
+        try:
+            out = np.ctypeslib.as_array(p, shape=(max(1, n.value),))[: n.value].copy()
+        finally:
+            L.oracle_free_buf(C.cast(p, C.c_void_p))
+        return st.value, out
     pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
     tri = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
     r = _as_res(res)

@@ -971,4 +971,25 @@ uint64_t oracle_voxelize(const float* points, uint64_t n_points, const int32_t* 
   return v.size();
 }
 
+uint64_t* oracle_voxelize_alloc(const float* points, uint64_t n_points, const int32_t* tris,
+                                uint64_t n_tris, const int32_t* res, int32_t key_bits,
+                                int32_t n_threads, uint64_t* n_out, int32_t* status) {
+  int st = validate(points, n_points, tris, n_tris, res, key_bits);
+  Grid g;
+  if (!st) st = make_grid(points, n_points, res, g);
+  std::vector<uint64_t> v;
+  if (!st) st = voxelize_all(points, tris, n_tris, g, key_bits, n_threads, v);
+  if (status) *status = st;
+  *n_out = 0;
+  if (st && st != ORC_E_EMPTY) return nullptr;
+  uint64_t* out = (uint64_t*)std::malloc(sizeof(uint64_t) * (v.size() ? v.size() : 1));
+  if (!out) {
+    if (status) *status = ORC_E_OOM;
+    return nullptr;
+  }
+  std::memcpy(out, v.data(), sizeof(uint64_t) * v.size());
+  *n_out = v.size();
+  return out;
+}
+
 }  // extern "C"

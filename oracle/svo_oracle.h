@@ -112,6 +112,12 @@ uint64_t oracle_voxelize(const float* points, uint64_t n_points, const int32_t* 
                          uint64_t n_tris, const int32_t* res, int32_t key_bits, uint64_t* dst,
                          uint64_t cap, int32_t* status);
 
+/* Voxelize only, on n_threads threads (<= 0: all); returns a malloc'd array
+ * of *n_out sorted unique codes (free with oracle_free_buf). */
+uint64_t* oracle_voxelize_alloc(const float* points, uint64_t n_points, const int32_t* tris,
+                                uint64_t n_tris, const int32_t* res, int32_t key_bits,
+                                int32_t n_threads, uint64_t* n_out, int32_t* status);
+
 #ifdef __cplusplus
 }
 #endif
