@@ -81,6 +81,7 @@ const char* svo_status_str(int s) {
     case SVO_E_EMPTY: return "empty mesh or no voxel";
     case SVO_E_BUSY: return "server queue full";
     case SVO_E_RANGE: return "destination buffer too small";
+    case SVO_E_TOO_LARGE: return "input beyond the fixed scratch of a no-allocation entry point";
     default: return "unknown status";
   }
 }

@@ -1,6 +1,7 @@
 // kernels_api.cpp -- include/svo/svo_kernels.h: single kernels on host buffers.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "svo/svo_kernels.h"
@@ -22,6 +23,44 @@ struct DevBufs {
     return p;
   }
 };
+
+// Fixed device scratch of svo_k_radix_sort_noalloc: the key count the sort
+// kernels read, the ping-pong key buffer and the digit counts of up to
+// NOALLOC_SORT_MAX_KEYS keys of 8 bytes.  Each device holds its own copy for
+// the life of the process.  In host code the array's name is not a device
+// address: kernels get the per-device address that hipGetSymbolAddress
+// returned for it (noalloc_scratch below).
+constexpr uint64_t NOALLOC_SORT_MAX_KEYS = 1ull << 22;
+constexpr size_t NOALLOC_SCRATCH_BYTES = 34ull << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+__global__ void k_set_count(unsigned long long* c, unsigned long long v) {
+  if (threadIdx.x == 0) *c = v;
+}
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // per device; resolved on its first call
+
+// Device address of `scratch` on `dev` (the current device).
+int noalloc_scratch(int dev, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr.empty()) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SVO_E_HIP;
+    g_scratch_ptr.assign((size_t)n, nullptr);
+  }
+  if (dev < 0 || (size_t)dev >= g_scratch_ptr.size()) return SVO_E_INVALID;
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return SVO_E_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SVO_OK;
+}
 
 }  // namespace
 
@@ -63,6 +102,51 @@ int svo_k_radix_sort(int device, void* keys, uint64_t n, int key_bytes, int bits
       st = SVO_E_HIP;
   }
   (void)hipStreamDestroy(s);
+  return st;
+}
+
+int svo_k_radix_sort_noalloc(int device, void* keys, uint64_t n, int key_bytes, int bits,
+                             void* stream) {
+  if ((key_bytes != 4 && key_bytes != 8) || bits < 0 || bits > 8 * key_bytes || (n && !keys))
+    return SVO_E_INVALID;
+  if (n > NOALLOC_SORT_MAX_KEYS) return SVO_E_TOO_LARGE;
+  if (n == 0) return SVO_OK;
+  // layout: [count | ping-pong keys | digit counts], each on a 256-byte boundary
+  const size_t o_keys = 256, o_tmp = o_keys + 8 * NOALLOC_SORT_MAX_KEYS;
+  if (o_tmp + radix_sort_tmp_bytes(NOALLOC_SORT_MAX_KEYS) > NOALLOC_SCRATCH_BYTES)
+    return SVO_E_TOO_LARGE;
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) return SVO_E_HIP;
+  int st = SVO_OK;
+  void* base = nullptr;
+  st = noalloc_scratch(device, &base);
+  if (st == SVO_OK) {
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long* cnt = (unsigned long long*)base;
+    void* pong = (char*)base + o_keys;
+    void* tmp = (char*)base + o_tmp;
+    const uint32_t blocks = Tuning{}.sort_blocks;
+    hipLaunchKernelGGL(k_set_count, dim3(1), dim3(64), 0, s, cnt, (unsigned long long)n);
+    void* out = nullptr;
+    if (key_bytes == 4) {
+      uint32_t* r = nullptr;
+      launch_radix_sort<uint32_t>((uint32_t*)keys, (uint32_t*)pong, cnt, n, bits, blocks, tmp,
+                                  &r, s);
+      out = r;
+    } else {
+      uint64_t* r = nullptr;
+      launch_radix_sort<uint64_t>((uint64_t*)keys, (uint64_t*)pong, cnt, n, bits, blocks, tmp,
+                                  &r, s);
+      out = r;
+    }
+    if (hipGetLastError() != hipSuccess) st = SVO_E_HIP;
+    // an odd number of passes leaves the keys in the scratch buffer
+    if (st == SVO_OK && out != keys &&
+        hipMemcpyAsync(keys, out, (size_t)key_bytes * n, hipMemcpyDeviceToDevice, s) != hipSuccess)
+      st = SVO_E_HIP;
+    if (st == SVO_OK && !stream && hipStreamSynchronize(s) != hipSuccess) st = SVO_E_HIP;
+  }
+  (void)hipSetDevice(prev);
   return st;
 }
 

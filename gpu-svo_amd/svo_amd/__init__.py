@@ -25,6 +25,7 @@ LIB_PATH = os.environ.get("SVO_AMD_LIB", os.path.join(PKG_ROOT, "libsvo_amd.so")
 
 OK, E_INVALID, E_RES_LIMIT, E_OOM, E_HIP, E_IO, E_PARSE, E_EMPTY, E_BUSY, E_RANGE = (
     0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
+E_TOO_LARGE = -10
 SEM_REFERENCE, SEM_INTENDED = 0, 1
 BUILD_NO_VERTEX_EDGE = 1
 
@@ -141,6 +142,8 @@ _SIGS = {
     "svo_server_destroy": (C.c_int, [C.c_void_p]),
     # include/svo/svo_kernels.h
     "svo_k_radix_sort": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_int, C.c_int]),
+    "svo_k_radix_sort_noalloc": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_int, C.c_int,
+                                           C.c_void_p]),
     "svo_k_voxelize": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
                                  C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
                                  C.POINTER(C.c_uint64)]),

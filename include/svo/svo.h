@@ -49,7 +49,8 @@ typedef enum {
   SVO_E_PARSE = -6,     /* OBJ line of 255+ characters (the reference loops forever on it)   */
   SVO_E_EMPTY = -7,     /* no points, no faces, or no voxel hit                              */
   SVO_E_BUSY = -8,      /* server queue full                                                 */
-  SVO_E_RANGE = -9      /* destination buffer too small                                      */
+  SVO_E_RANGE = -9,     /* destination buffer too small                                      */
+  SVO_E_TOO_LARGE = -10 /* input beyond the fixed scratch of a no-allocation entry point     */
 } svo_status;
 
 /* Output semantics.  REFERENCE reproduces the reference's arrays bit for bit,

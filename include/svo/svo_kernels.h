@@ -22,6 +22,21 @@ extern "C" {
 /* Sorts n keys (key_bytes 4 or 8) on bits [0, bits) in place. */
 int svo_k_radix_sort(int device, void* keys, uint64_t n, int key_bytes, int bits);
 
+/* Sorts n device-resident keys (key_bytes 4 or 8) on bits [0, bits) in place
+ * with the kernels of svo_k_radix_sort, allocating nothing: the key count,
+ * the ping-pong buffer and the digit counts live in a fixed scratch array of
+ * 34 MiB that each device holds from its first call until the process ends.
+ * No counterpart in the reference; for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops).  n is at most
+ * 4,194,304 (2^22) keys; larger n returns SVO_E_TOO_LARGE.  `stream` is a
+ * hipStream_t of `device`: the call only enqueues on it, or, when it is
+ * NULL, runs on the device's null stream and returns when the keys are
+ * sorted.  All calls on one device share its scratch, so calls whose work
+ * overlaps on the device (other streams, other threads) race.  The caller's
+ * current device is restored on return. */
+int svo_k_radix_sort_noalloc(int device, void* keys, uint64_t n, int key_bytes, int bits,
+                             void* stream);
+
 /* Voxelizes one mesh: writes up to cap hit codes (uint32 for key_bits 32,
  * uint64 for 64) in emission order, duplicates included; n_hits receives the
  * number of hits (may exceed cap). */
