@@ -188,6 +188,10 @@ int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap,
 
 uint64_t svo_ctx_bytes_in_use(const svo_ctx* ctx) { return ctx ? ctx->impl->in_use.load() : 0; }
 
+uint64_t svo_ctx_workspace_bytes(const svo_ctx* ctx) {
+  return ctx ? ctx->impl->ws_bytes.load() : 0;
+}
+
 int svo_ctx_last_pass_stats(svo_ctx* ctx, svo_pass_stats* out) {
   if (!ctx || !out) return SVO_E_INVALID;
   std::lock_guard<std::mutex> lk(ctx->impl->stage_mu);
@@ -249,6 +253,10 @@ int svo_validate(const svo_mesh_desc* mesh, const svo_params* params) {
 
 uint64_t svo_estimate_bytes(uint64_t n_points, uint64_t n_tris, const svo_params* params) {
   return params ? estimate_bytes(n_points, n_tris, params) : 0;
+}
+
+uint64_t svo_estimate_bytes_mesh(const svo_mesh_desc* mesh, const svo_params* params) {
+  return mesh && params ? estimate_bytes_mesh(mesh, params) : 0;
 }
 
 static int build_common(svo_ctx* ctx, const svo_mesh_desc* mesh, const svo_params* params,

@@ -207,7 +207,7 @@ int Workspace::get(const char* name, size_t bytes, void** out) {
   if (b.bytes < bytes || oversize) {
     if (b.ptr) {
       ctx->dev_free(b.ptr, b.bytes);
-      ctx->ws_bytes.fetch_sub(b.bytes);
+      ctx->ws_bytes.fetch_sub(Context::size_class(b.bytes));
     }
     b.ptr = nullptr;
     b.bytes = 0;
@@ -224,7 +224,7 @@ int Workspace::get(const char* name, size_t bytes, void** out) {
       b.bytes = want;
     }
     b.ptr = p;
-    ctx->ws_bytes.fetch_add(b.bytes);
+    ctx->ws_bytes.fetch_add(Context::size_class(b.bytes));  // as in_use counts it
   }
   *out = b.ptr;
   return SVO_OK;
@@ -234,7 +234,7 @@ void Workspace::release_all() {
   for (auto& kv : bufs)
     if (kv.second.ptr) {
       ctx->dev_free(kv.second.ptr, kv.second.bytes);
-      ctx->ws_bytes.fetch_sub(kv.second.bytes);
+      ctx->ws_bytes.fetch_sub(Context::size_class(kv.second.bytes));
     }
   bufs.clear();
 }
@@ -299,6 +299,321 @@ uint64_t estimate_bytes(uint64_t n_points, uint64_t n_tris, const svo_params* p)
   const double b = nodes * per_node + hits * 8.0 * 2.5 + (double)n_points * 12.0 +
                    (double)n_tris * 12.0 + (1 << 20);
   return (uint64_t)b;
+}
+
+// Upper bounds on the device bytes one build takes on a fresh lane: its
+// output block plus every workspace buffer of the pass, with the growth
+// headroom and size-class rounding of the allocator, for either
+// de-duplication path.  `U` are bounds on the level code counts U_1..U_L,
+// `cells` on the candidate cells (box cells summed over the triangles) and
+// `rows` on their box rows.
+namespace {
+double pass_bytes_bound(const svo_params* p, int kb, uint64_t P, uint64_t T, int levels,
+                        const std::vector<double>& U, double cells, double rows) {
+  auto cells_at = [&](int l) {
+    double c = 1.0;
+    for (int a = 0; a < 3; ++a)
+      c *= (double)(((int64_t)p->res[a] + (1ll << (l - 1)) - 1) >> (l - 1));
+    return c;
+  };
+  const double K = kb == 64 ? 8.0 : 4.0;
+  double N = 1.0;
+  for (int l = 2; l <= levels; ++l) N += 8.0 * U[l];
+  // output block (run_typed's layout)
+  const bool attrs = !(p->flags & SVO_BUILD_NO_VERTEX_EDGE);
+  const double rec =
+      attrs ? (16.0 * 8 + 28.0 * 12 + (p->semantics == SVO_SEM_INTENDED ? 28.0 * 6 : 0)) : 0.0;
+  const double out = (K + 161.0 + rec) * N + K * U[1] + 12 * 256.0;
+  // workspace
+  const uint64_t bm_words =
+      ((uint64_t)morton3<uint64_t>((uint32_t)p->res[0] - 1, (uint32_t)p->res[1] - 1,
+                                   (uint32_t)p->res[2] - 1) + 1 + 31) / 32;
+  const double key_cap = std::max(cells, 1.0);
+  const uint64_t kc = (uint64_t)std::min(key_cap, 4.0e18);
+  double lv = 0.0, cap = key_cap;
+  for (int l = 1; l <= levels + 1; ++l) {
+    if (l >= 2) cap = std::min(cap, cells_at(l));
+    lv += K * cap + (l >= 2 ? 8.0 * cap : 0.0) + 3 * 256.0;
+  }
+  const double n_big = (double)T + rows / 512.0;
+  const double ws_common = 12.0 * P + 12.0 * T + 2 * 256.0 + lv + (double)level_tmp_bytes(kc) +
+                           sizeof(BigItem) * std::max(n_big, 1.0) + (double)attr_tmp_bytes(4096) +
+                           (1 << 20);  // control buffers of one mesh
+  const double ws_bitmap = 4.0 * bm_words + (double)bitmap_level_tmp_bytes(bm_words);
+  const double ws_sort = 2.0 * K * key_cap + (double)radix_sort_tmp_bytes(kc);
+  const double ws = ws_common + std::max(ws_bitmap, ws_sort);
+  // allocator: 1/8 growth headroom on workspace buffers, size classes <= 1.25x
+  return 1.25 * out + 1.25 * 1.125 * ws + (2 << 20);
+}
+
+int levels_of(const svo_params* p) {
+  int levels = 1;  // levels until every axis has one cell
+  while (true) {
+    bool one = true;
+    for (int a = 0; a < 3; ++a)
+      if (((int64_t)p->res[a] + (1ll << (levels - 1)) - 1) >> (levels - 1) > 1) one = false;
+    if (one) return levels;
+    ++levels;
+  }
+}
+
+double cells_of_level(const svo_params* p, int l) {
+  double c = 1.0;
+  for (int a = 0; a < 3; ++a) c *= (double)(((int64_t)p->res[a] + (1ll << (l - 1)) - 1) >> (l - 1));
+  return c;
+}
+
+// Morton code of coordinates below 2^16 by byte lookup (same value as morton3)
+struct MortonLut {
+  uint32_t spread[256];
+  MortonLut() {
+    for (uint32_t b = 0; b < 256; ++b) {
+      uint32_t v = 0;
+      for (int i = 0; i < 8; ++i) v |= ((b >> i) & 1u) << (3 * i);
+      spread[b] = v;
+    }
+  }
+  uint64_t sp(uint32_t c) const {
+    return (uint64_t)spread[c & 255] | ((uint64_t)spread[(c >> 8) & 255] << 24);
+  }
+  uint64_t operator()(uint32_t x, uint32_t y, uint32_t z) const {
+    return sp(x) | (sp(y) << 1) | (sp(z) << 2);
+  }
+};
+const MortonLut g_morton_lut;
+}  // namespace
+
+// The bound from the grid alone: every level full (U_l = cells of level l),
+// and candidate cells no more than the automatic path choice allows on the
+// sort path (it sorts only when the bitmap is over 64 bytes per candidate
+// cell; the bitmap path keeps at most one key per grid cell).  Not a bound
+// when a context forces the sort path.
+uint64_t grid_bound_bytes(const svo_params* p, int kb, uint64_t n_points, uint64_t n_tris) {
+  const int levels = levels_of(p);
+  std::vector<double> U(levels + 2, 0.0);
+  for (int l = 1; l <= levels; ++l) U[l] = cells_of_level(p, l);
+  const double bm_bytes = 4.0 * (double)(((uint64_t)morton3<uint64_t>(
+                                              (uint32_t)p->res[0] - 1, (uint32_t)p->res[1] - 1,
+                                              (uint32_t)p->res[2] - 1) + 1 + 31) / 32);
+  const double rows = (double)n_tris * (double)p->res[1] * (double)p->res[2];
+  const double b = pass_bytes_bound(p, kb, n_points, n_tris, levels, U,
+                                    std::min(U[1], bm_bytes / 64.0), rows);
+  return b >= 1.8e19 ? UINT64_MAX : (uint64_t)b;
+}
+
+// The bound from the mesh: exact where the pass's sizes follow from the
+// triangles alone (the grid, each triangle's box of candidate cells, hence
+// the key capacity and the bitmap size); for the data-dependent counts
+//   voxels U_1 <= min(candidate cells, grid cells, sum over triangles of the
+//     cells their plane can cross: a triangle whose normal in cell units m
+//     has dominant axis d crosses at most floor((|m_a| + |m_b|) / |m_d|) + 2
+//     cells of a column along d, one more here for rounding),
+//   level-l codes U_l <= min(U_(l-1), grid cells of level l, that per-triangle
+//     sum at level l for the fine levels, and from a level k on the union of
+//     the level-k cells that the triangles' planes can touch inside their
+//     boxes),
+//   nodes N <= 1 + 8 sum_(l>=2) U_l.
+// A hit voxel lies in its triangle's box and touches its plane, so its level-k
+// ancestor touches the plane inside the box's level-k image.  The union is
+// counted on a level-k occupancy bitmap in Morton order, whose nonzero bytes
+// are the level-(k+1) union, and so on up; k is the finest level at which the
+// triangles' boxes hold few enough cells that marking stays cheap.
+// Falls back to the count-only estimate for inputs the build rejects.
+uint64_t estimate_bytes_mesh(const svo_mesh_desc* m, const svo_params* p) {
+  int kb = 32;
+  if (validate_params(m, p, &kb) != SVO_OK)
+    return m && p ? estimate_bytes(m->n_points, m->n_tris, p) : 0;
+  const uint64_t P = m->n_points, T = m->n_tris;
+  // grid: createMeshBBCube (BaseModel.h:24-49) and the voxel size
+  // (SVO.cu:348-350), in the same float operations as k_params
+  float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint64_t i = 0; i < P; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const float v = m->xyz[3 * i + a];
+      if (!std::isfinite(v)) return estimate_bytes(P, T, p);
+      mn[a] = fmin_r(v, mn[a]);
+      mx[a] = fmax_r(v, mx[a]);
+    }
+  float len[3], lo[3], hi[3], unit[3];
+  for (int a = 0; a < 3; ++a) {
+    len[a] = mx[a] - mn[a];
+    lo[a] = mn[a];
+    hi[a] = mx[a];
+  }
+  const float L = fmax_r(len[0], fmax_r(len[1], len[2]));
+  if (!(L > 0.0f)) return estimate_bytes(P, T, p);
+  for (int a = 0; a < 3; ++a)
+    if (L != len[a]) {
+      const float delta = L - len[a];
+      lo[a] = mn[a] - (delta / 2.0f);
+      hi[a] = mx[a] + (delta / 2.0f);
+    }
+  for (int a = 0; a < 3; ++a) {
+    const float eps = (hi[a] - lo[a]) / 10001.0f;
+    lo[a] -= eps;
+    hi[a] += eps;
+    unit[a] = (hi[a] - lo[a]) / (float)p->res[a];
+  }
+  // Each point in cells: (p - gridOrigin) / unit, rounded as the voxelizer
+  // rounds it.  Rounded division by a positive unit is monotone, so the
+  // voxelizer's (int)(min_k v_k / unit) is (int)(min_k q_k): the triangle
+  // boxes below are the voxelizer's exactly (SVO.cu:73-85).
+  std::vector<float> q(3 * P);
+  for (uint64_t i = 0; i < P; ++i)
+    for (int a = 0; a < 3; ++a) q[3 * i + a] = (m->xyz[3 * i + a] - lo[a]) / unit[a];
+  const int levels = levels_of(p);
+  const int rmax[3] = {p->res[0] - 1, p->res[1] - 1, p->res[2] - 1};
+  // pass 1: boxes, candidate cells and rows, box cells per level
+  std::vector<int32_t> box(6 * T);
+  std::vector<double> boxsum(levels + 2, 0.0);
+  double cells = 0.0, rows = 0.0;
+  for (uint64_t t = 0; t < T; ++t) {
+    const int32_t* ti = m->tri + 3 * t;
+    if (ti[0] < 0 || ti[1] < 0 || ti[2] < 0 || (uint64_t)ti[0] >= P || (uint64_t)ti[1] >= P ||
+        (uint64_t)ti[2] >= P)
+      return estimate_bytes(P, T, p);  // the build rejects the mesh
+    const float* v0 = &q[3 * (uint64_t)ti[0]];
+    const float* v1 = &q[3 * (uint64_t)ti[1]];
+    const float* v2 = &q[3 * (uint64_t)ti[2]];
+    int32_t* g = &box[6 * t];
+    for (int a = 0; a < 3; ++a) {
+      g[a] = clamp_i((int)fmin_r(v0[a], fmin_r(v1[a], v2[a])), 0, rmax[a]);
+      g[3 + a] = clamp_i((int)fmax_r(v0[a], fmax_r(v1[a], v2[a])), 0, rmax[a]);
+    }
+    const double e1 = (double)(g[3] - g[0] + 1) * (g[4] - g[1] + 1) * (g[5] - g[2] + 1);
+    cells += e1;
+    rows += (double)(g[4] - g[1] + 1) * (g[5] - g[2] + 1);
+    boxsum[1] += e1;
+    for (int l = 2; l <= levels; ++l) {
+      const int sh = l - 1;
+      boxsum[l] += (double)((g[3] >> sh) - (g[0] >> sh) + 1) * ((g[4] >> sh) - (g[1] >> sh) + 1) *
+                   ((g[5] >> sh) - (g[2] >> sh) + 1);
+    }
+  }
+  // union level: the finest level >= 2 whose boxes hold at most 32 cells per
+  // triangle on average (or 4M cells in all), and whose bitmap fits the byte
+  // lookup and 256 MB
+  int k = 0;
+  uint64_t bits_k = 0;
+  for (int l = 2; l <= levels && !k; ++l) {
+    if (boxsum[l] > std::max(32.0 * (double)T, 4194304.0)) continue;
+    uint32_t r[3];
+    for (int a = 0; a < 3; ++a) r[a] = (uint32_t)(((int64_t)p->res[a] + (1ll << (l - 1)) - 1) >> (l - 1));
+    if (std::max({r[0], r[1], r[2]}) > 65536u) continue;
+    const uint64_t bits = morton3<uint64_t>(r[0] - 1, r[1] - 1, r[2] - 1) + 1;
+    if (bits > (1ull << 31)) continue;
+    k = l;
+    bits_k = bits;
+  }
+  const int fine_end = k ? k - 1 : levels;  // per-triangle sums on levels 1..fine_end
+  std::vector<uint8_t> occ;
+  if (k) occ.assign((size_t)((bits_k + 7) / 8 + 8), 0);
+  const int ks = k ? k - 1 : 0;  // level-k cell = voxel >> ks
+  const double inv_s = 1.0 / (double)(1ll << ks);
+  std::vector<double> lvl_bound(levels + 2, INFINITY);
+  for (int l = 1; l <= fine_end; ++l) lvl_bound[l] = 0.0;
+  // pass 2: plane limits on the fine levels, marks on level k
+  for (uint64_t t = 0; t < T; ++t) {
+    const int32_t* g = &box[6 * t];
+    const int ex[3] = {g[3] - g[0], g[4] - g[1], g[5] - g[2]};
+    bool slab = false;
+    int d = 0, a1 = 1, a2 = 2;
+    double per_col = INFINITY, ga = 0.0, gb = 0.0, z0 = 0.0, x0 = 0.0, y0 = 0.0;
+    if (ex[0] > 2 || ex[1] > 2 || ex[2] > 2) {
+      // plane normal in cell units (boxes of at most 3 cells per axis are
+      // counted whole: their plane limits little)
+      const int32_t* ti = m->tri + 3 * t;
+      const float* v0 = &q[3 * (uint64_t)ti[0]];
+      const float* v1 = &q[3 * (uint64_t)ti[1]];
+      const float* v2 = &q[3 * (uint64_t)ti[2]];
+      const double e0[3] = {(double)v1[0] - v0[0], (double)v1[1] - v0[1], (double)v1[2] - v0[2]};
+      const double e1[3] = {(double)v2[0] - v1[0], (double)v2[1] - v1[1], (double)v2[2] - v1[2]};
+      const double mm[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2],
+                            e0[0] * e1[1] - e0[1] * e1[0]};
+      const double am[3] = {std::fabs(mm[0]), std::fabs(mm[1]), std::fabs(mm[2])};
+      d = am[0] >= am[1] ? (am[0] >= am[2] ? 0 : 2) : (am[1] >= am[2] ? 1 : 2);
+      a1 = d == 2 ? 0 : d + 1;
+      a2 = d == 0 ? 2 : (d == 1 ? 0 : 1);
+      // the voxelizer's float normal can point elsewhere on a sliver (edges
+      // nearly parallel, |e0 x e1| < 1e-3 |e0| |e1|): no plane limit then
+      const double l0 = e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2];
+      const double l1 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+      const double lc = am[0] * am[0] + am[1] * am[1] + am[2] * am[2];
+      slab = am[d] > 0.0 && lc > 1e-6 * l0 * l1;
+      if (slab) {
+        const double inv_d = 1.0 / mm[d];
+        per_col = std::floor((am[a1] + am[a2]) * std::fabs(inv_d)) + 3.0;
+        ga = -mm[a1] * inv_d;
+        gb = -mm[a2] * inv_d;
+        z0 = v0[d] * inv_s;
+        x0 = v0[a1] * inv_s;
+        y0 = v0[a2] * inv_s;
+      }
+    }
+    for (int l = 1; l <= fine_end; ++l) {
+      const int sh = l - 1;
+      const double e[3] = {(double)((g[3] >> sh) - (g[0] >> sh) + 1),
+                           (double)((g[4] >> sh) - (g[1] >> sh) + 1),
+                           (double)((g[5] >> sh) - (g[2] >> sh) + 1)};
+      lvl_bound[l] += std::min(e[0] * e[1] * e[2], e[a1] * e[a2] * std::min(e[d], per_col));
+    }
+    if (!k) continue;
+    int b0[3], b1[3];
+    for (int a = 0; a < 3; ++a) {
+      b0[a] = g[a] >> ks;
+      b1[a] = g[3 + a] >> ks;
+    }
+    uint32_t c[3];
+    for (int j = b0[a2]; j <= b1[a2]; ++j) {
+      c[a2] = (uint32_t)j;
+      for (int i = b0[a1]; i <= b1[a1]; ++i) {
+        c[a1] = (uint32_t)i;
+        int k0 = b0[d], k1 = b1[d];
+        if (slab) {
+          // the level-k cells of this column that the plane can touch: its
+          // range over the column's corners, widened by a quarter cell for
+          // rounding (the voxelizer's float plane differs from this one by
+          // far less on a triangle that is not a sliver)
+          const double zc = z0 + ga * (i - x0) + gb * (j - y0);
+          const double zl = zc + std::min(0.0, ga) + std::min(0.0, gb);
+          const double zh = zc + std::max(0.0, ga) + std::max(0.0, gb);
+          k0 = (int)std::max((double)b0[d], std::floor(zl - 0.25));
+          k1 = (int)std::min((double)b1[d], std::floor(zh + 0.25));
+        }
+        for (int kk = k0; kk <= k1; ++kk) {
+          c[d] = (uint32_t)kk;
+          const uint64_t code = g_morton_lut(c[0], c[1], c[2]);
+          occ[code >> 3] |= (uint8_t)(1u << (code & 7));
+        }
+      }
+    }
+  }
+  // union counts of levels k, k+1, ...: set bits, then each 8-byte word's
+  // nonzero bytes as one byte of the next level
+  for (int l = k; k && l <= levels; ++l) {
+    const size_t nw = (occ.size() + 7) / 8;
+    occ.resize(8 * nw, 0);
+    uint64_t set = 0;
+    std::vector<uint8_t> up(nw, 0);
+    for (size_t i = 0; i < nw; ++i) {
+      uint64_t w;
+      std::memcpy(&w, occ.data() + 8 * i, 8);
+      set += (uint64_t)__builtin_popcountll(w);
+      uint64_t tt = w | (w >> 4);
+      tt |= tt >> 2;
+      tt |= tt >> 1;
+      tt &= 0x0101010101010101ull;
+      up[i] = (uint8_t)((tt * 0x0102040810204080ull) >> 56);
+    }
+    lvl_bound[l] = (double)set;
+    occ.swap(up);
+  }
+  std::vector<double> U(levels + 2, 0.0);
+  U[1] = std::min({std::max(cells, 1.0), cells_of_level(p, 1), lvl_bound[1]});
+  for (int l = 2; l <= levels; ++l)
+    U[l] = std::min({U[l - 1], cells_of_level(p, l), lvl_bound[l]});
+  const double b = pass_bytes_bound(p, kb, P, T, levels, U, cells, rows);
+  return b >= 1.8e19 ? UINT64_MAX : (uint64_t)b;
 }
 
 // ------------------------------------------------------------------ timing

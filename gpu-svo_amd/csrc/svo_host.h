@@ -150,6 +150,8 @@ int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree
 
 int validate_params(const svo_mesh_desc* mesh, const svo_params* p, int* key_bits_out);
 uint64_t estimate_bytes(uint64_t n_points, uint64_t n_tris, const svo_params* p);
+uint64_t estimate_bytes_mesh(const svo_mesh_desc* m, const svo_params* p);
+uint64_t grid_bound_bytes(const svo_params* p, int key_bits, uint64_t n_points, uint64_t n_tris);
 
 // OBJ I/O (obj_io.cpp).
 int load_obj_file(const char* path, std::vector<float>& pts, std::vector<int32_t>& tris);

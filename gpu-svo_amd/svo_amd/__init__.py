@@ -100,6 +100,7 @@ _SIGS = {
                                         C.POINTER(C.c_char_p), C.c_int,
                                         C.POINTER(C.c_int)]),
     "svo_ctx_bytes_in_use": (C.c_uint64, [C.c_void_p]),
+    "svo_ctx_workspace_bytes": (C.c_uint64, [C.c_void_p]),
     "svo_device_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "svo_device_free": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "svo_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]),
@@ -137,6 +138,7 @@ _SIGS = {
     "svo_server_poll": (C.c_int, [C.c_void_p, C.POINTER(Completion), C.c_uint32,
                                   C.POINTER(C.c_uint32), C.c_int64]),
     "svo_estimate_bytes": (C.c_uint64, [C.c_uint64, C.c_uint64, C.POINTER(Params)]),
+    "svo_estimate_bytes_mesh": (C.c_uint64, [C.POINTER(MeshDesc), C.POINTER(Params)]),
     "svo_server_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32),
                                    C.POINTER(C.c_uint32)]),
     "svo_server_destroy": (C.c_int, [C.c_void_p]),
@@ -241,6 +243,13 @@ def estimate_bytes(n_points, n_tris, res, semantics=SEM_REFERENCE, flags=0) -> i
     return int(lib().svo_estimate_bytes(n_points, n_tris, C.byref(p)))
 
 
+def estimate_bytes_mesh(points, tris, res, semantics=SEM_REFERENCE, key_bits=0, flags=0) -> int:
+    """Upper bound on the device bytes one build of the mesh takes."""
+    d, _keep = _host_desc(points, tris)
+    p = make_params(res, semantics, key_bits, flags)
+    return int(lib().svo_estimate_bytes_mesh(C.byref(d), C.byref(p)))
+
+
 _FIELD_DT = {
     "is_leaf": (np.uint8, None), "origin": (np.float32, 3), "width": (np.float32, None),
     "parent": (np.uint32, None), "childs": (np.uint32, 8), "neighbors": (np.uint32, 27),
@@ -308,6 +317,33 @@ class Tree:
 
     def device_ptr(self, name: str) -> int:
         return lib().svo_tree_device_ptr(self._h, FIELDS[name]) or 0
+
+    def field_rows(self, name: str, start: int, count: int) -> np.ndarray:
+        """Rows [start, start + count) of a node or record field (a slice of
+        what field() returns), copied straight from device memory."""
+        if name in ("voxels", "code"):
+            dt, width = (np.uint64 if self.info.key_bits == 64 else np.uint32), None
+        else:
+            dt, width = _FIELD_DT[name]
+            if name in ("level_nodes", "nodes_aos"):
+                raise ValueError(f"{name} has no device rows")
+        dt = np.dtype(dt)
+        row = dt.itemsize * (width or 1)
+        L = lib()
+        nbytes = int(L.svo_tree_field_bytes(self._h, FIELDS[name]))
+        if name in ("voxels", "code"):
+            n = self.info.n_voxels if name == "voxels" else self.info.n_nodes
+            if nbytes != n * row:
+                raise ValueError(f"{name}: stored width differs from key_bits")
+        total = nbytes // row
+        start = max(0, int(start))
+        count = max(0, min(int(count), total - start))
+        a = np.empty(count * (width or 1), dtype=dt)
+        if count:
+            src = self.device_ptr(name) + start * row
+            _check(L.svo_device_copy(self._ctx.handle, a.ctypes.data, C.c_void_p(src),
+                                     count * row, 1), f"copy rows of {name}")
+        return a.reshape(-1, width) if width else a
 
     def write_obj(self, directory: str, stem: str) -> str:
         buf = C.create_string_buffer(4096)
@@ -387,6 +423,10 @@ class Context:
 
     def bytes_in_use(self) -> int:
         return int(lib().svo_ctx_bytes_in_use(self._h))
+
+    def workspace_bytes(self) -> int:
+        """The part of bytes_in_use() held by the lanes' workspaces."""
+        return int(lib().svo_ctx_workspace_bytes(self._h))
 
     def upload_mesh(self, points, tris) -> "DeviceMesh":
         """Copies a mesh to device memory (kept until DeviceMesh.free)."""

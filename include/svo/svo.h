@@ -195,6 +195,9 @@ typedef struct {
 } svo_pass_stats;
 int svo_ctx_last_pass_stats(svo_ctx* ctx, svo_pass_stats* out);
 uint64_t svo_ctx_bytes_in_use(const svo_ctx* ctx);
+/* The part of svo_ctx_bytes_in_use held by the pipeline lanes' workspaces,
+ * which are kept between builds; the rest is trees and svo_device_alloc. */
+uint64_t svo_ctx_workspace_bytes(const svo_ctx* ctx);
 /* Device memory on the context's device (counted against its budget), for
  * callers that keep meshes resident for svo_build_device / svo_build_batch. */
 int svo_device_alloc(svo_ctx* ctx, uint64_t bytes, void** out);
@@ -289,8 +292,14 @@ int svo_server_submit(svo_server* srv, const svo_mesh_desc* mesh, const svo_para
 /* Returns up to cap completions; waits up to timeout_us for the first one. */
 int svo_server_poll(svo_server* srv, svo_completion* out, uint32_t cap, uint32_t* n,
                     int64_t timeout_us);
-/* Host-side device-memory estimate used for admission. */
+/* Quick device-memory estimate from the counts alone, for a closed surface
+ * spanning its box (not a bound: a triangle soup can take far more). */
 uint64_t svo_estimate_bytes(uint64_t n_points, uint64_t n_tris, const svo_params* params);
+/* Upper bound on the device bytes one build of the mesh takes on a fresh
+ * context (output plus every workspace buffer, allocator rounding included),
+ * from one host pass over its points and triangles; the served path admits
+ * requests by it.  Inputs the build rejects get svo_estimate_bytes. */
+uint64_t svo_estimate_bytes_mesh(const svo_mesh_desc* mesh, const svo_params* params);
 int svo_server_stats(svo_server* srv, uint64_t* bytes_reserved, uint32_t* pending,
                      uint32_t* in_flight);
 int svo_server_destroy(svo_server* srv);

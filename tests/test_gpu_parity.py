@@ -216,3 +216,38 @@ def test_named_readouts(gpu_ctx, bunny):
             assert buf.tobytes() == ref.tobytes(), name
     finally:
         tree.free()
+
+
+def test_reference_unique_removes_records(gpu_ctx):
+    """The reference's consecutive unique (uniqueVert / uniqueEdge,
+    SVO.cu:679-720) removes edge records where node corners collapse in
+    float32: icospheres far from the origin (o + w == o for widths below
+    2^-8 at 1e5).  The build takes the count / scan / write path there;
+    alone and in a batch beside ordinary meshes it must give the oracle's
+    arrays (the oracle's removals are pinned in test_oracle_pins.py)."""
+    from test_oracle_pins import far_mesh
+    cases = [(0.25, 256), (0.05, 64), (0.02, 100)]
+    batch = []
+    for radius, res in cases:
+        P, T = far_mesh(radius=radius)
+        print(f"far mesh radius {radius} res {res}", flush=True)
+        ref = po.build(P, T, res)
+        assert ref.status == 0
+        assert ref.info.n_edges < 12 * ref.info.n_nodes  # records were removed
+        tree = gpu_ctx.build(P, T, res)
+        try:
+            compare(tree, ref)
+        finally:
+            tree.free()
+        batch.append((P, T, res, ref))
+    Pn, Tn = _meshgen().icosphere(3)
+    batch.insert(1, (Pn, Tn, 64, po.build(Pn, Tn, 64)))
+    print(f"far meshes in a batch of {len(batch)}", flush=True)
+    trees, st = gpu_ctx.build_batch([(P, T, r) for P, T, r, _ in batch])
+    try:
+        assert all(s == 0 for s in st), st
+        for t, (_, _, _, ref) in zip(trees, batch):
+            compare(t, ref)
+    finally:
+        for t in trees:
+            t.free()

@@ -81,7 +81,7 @@ def test_served_budget_admission(gpu_ctx):
     # them while the caller keeps freeing finished trees
     import svo_amd
     reqs = _requests(12, seed=32)
-    est = max(svo_amd.estimate_bytes(P.shape[0], T.shape[0], r) for P, T, r in reqs)
+    est = max(svo_amd.estimate_bytes_mesh(P, T, r) for P, T, r in reqs)
     base = gpu_ctx.bytes_in_use()
     srv = svo_amd.Server(gpu_ctx, mem_budget=base + 3 * est + (64 << 20), max_batch=4, n_lanes=2,
                          max_wait_us=100)
@@ -132,3 +132,66 @@ def _drain_some(srv, timeout_s=10):
         if not got and srv.stats()["pending"] == 0 and srv.stats()["in_flight"] == 0:
             break
     return out
+
+
+def test_estimate_bounds_device_bytes():
+    """svo_estimate_bytes_mesh is at least what a build takes on a fresh
+    context (its tree plus the lane's workspace, allocator rounding
+    included), across res 64-512 meshes of the served mix, a large noisy
+    icosphere and the 2,000,000-triangle soup at res 1024."""
+    import svo_amd
+    from svo_amd import meshgen
+    cases = []
+    for s in meshgen.mixed_specs(24, seed=41, res_choices=(64, 128, 256, 512))[::3]:
+        P, T = s.build()
+        cases.append((f"{s.kind}{s.params} res {s.res}", P, T, s.res, 0))
+    P, T = meshgen.noisy_icosphere(6, seed=1)
+    cases.append(("icosphere-6 + noise res 512 intended", P, T, 512, 1))
+    P, T = meshgen.icosphere(2)
+    cases.append(("icosphere-2 res 512", P, T, 512, 0))
+    P, T = meshgen.soup(2_000_000, seed=2)
+    cases.append(("soup 2M res 1024", P, T, 1024, 0))
+    for name, P, T, res, sem in cases:
+        bound = svo_amd.estimate_bytes_mesh(P, T, res, semantics=sem)
+        ctx = svo_amd.Context(0)
+        try:
+            base = ctx.bytes_in_use()
+            print(f"estimate: {name}", flush=True)
+            tree = ctx.build(P, T, res, semantics=sem)
+            used = ctx.bytes_in_use() - base
+            print(f"  bound {bound / 1e9:.3f} GB, used {used / 1e9:.3f} GB "
+                  f"({bound / used:.2f}x)", flush=True)
+            assert bound >= used, name
+            tree.free()
+        finally:
+            ctx.close()
+
+
+def test_served_with_context_budget_completes():
+    """A context budget that holds only a few trees: passes of split batches
+    run out of memory and are retried (or complete with SVO_E_OOM); every
+    request completes, and once the trees are freed the context holds only
+    its lanes' workspaces (no tree of a failed pass leaks)."""
+    import svo_amd
+    reqs = _requests(64, seed=33)
+    est = max(svo_amd.estimate_bytes_mesh(P, T, r) for P, T, r in reqs)
+    ctx = svo_amd.Context(0, mem_budget=6 * est)
+    try:
+        srv = svo_amd.Server(ctx, max_batch=40, n_lanes=2, max_wait_us=2000)
+        try:
+            for P, T, res in reqs:
+                srv.submit(P, T, res)
+            done = _drain(srv, len(reqs))
+        finally:
+            srv.close()
+        st = [d.status for d in done.values()]
+        print(f"statuses {sorted(set(st))}, ok {st.count(0)} of {len(st)}", flush=True)
+        assert set(st) <= {0, svo_amd.E_OOM}
+        assert st.count(0) > 0
+        for d in done.values():
+            if d.tree:
+                d.tree.free()
+        print(f"in use {ctx.bytes_in_use()}, workspaces {ctx.workspace_bytes()}", flush=True)
+        assert ctx.bytes_in_use() == ctx.workspace_bytes()
+    finally:
+        ctx.close()
