@@ -13,8 +13,10 @@ per pass (the plain path) are timed beside it.  Output: one JSON line.
 `roofline` is computed for the kernel stage that takes the most device time,
 from HIP events recorded on the build's stream, and its algorithmic bytes.
 `cpu_baseline` times the CPU restatement of the reference (the oracle) on a
-bounded sample of the same meshes on this host, on up to 16 threads (the
-single-threaded figure is reported beside it).
+bounded sample of the same meshes on this host, on every core the process may
+use (the single-threaded figure, the host's CPU count and model beside it).
+A short served stream (Poisson arrivals, res 64-512) follows the batch and
+its p50 / p99 latency go into the same line.
 """
 from __future__ import annotations
 
@@ -123,12 +125,24 @@ def pass_stats(ctx):
 
 
 def cpu_threads():
-    """Host threads for the multi-threaded CPU baseline (at most 16)."""
+    """Host threads for the multi-threaded CPU baseline: every core this
+    process may run on."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(meshes, budget_s=12.0):
@@ -154,6 +168,7 @@ def cpu_baseline(meshes, budget_s=12.0):
                     meshes[:done]))
     tn = time.perf_counter() - t0
     return {"value": round(done / tn, 3), "unit": "meshes/s", "cores": nt, "kind": "port",
+            "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
             "single_thread_value": round(done / t1, 3),
             "sample": f"first {done} meshes of the same 256-mesh set, build only "
                       f"(voxelize -> vertex/edge): {t1:.1f} s on 1 thread, {tn:.1f} s wall "
@@ -169,7 +184,6 @@ def run_batch_workload(args):
     meshes = meshgen.mixed_batch(256, seed=3)
     n_tris = sum(int(T.shape[0]) for _, T, _ in meshes)
     ctx = svo_amd.Context(0)
-    ctx.set_timing(True)
     if args.batch_lanes:
         ctx.set_option("batch_lanes", args.batch_lanes)
     for opt in args.option:
@@ -209,16 +223,25 @@ def run_batch_workload(args):
     for i in range(args.warmup):
         log(f"bench: warmup batched {i}")
         batch_step()
-    stage_acc: dict = {}
+    # timed steps without stage events; the same number of steps with HIP
+    # events on every stage follows, for the stage times and the roofline
     log(f"bench: timing {args.steps} batched steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         nv, nn = batch_step()
+    t1 = time.perf_counter()
+    ms_batch = (t1 - t0) * 1e3 / args.steps
+    log(f"bench: {args.steps} batched steps with stage events")
+    ctx.set_timing(True)
+    stage_acc: dict = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch_step()
         for k, v in ctx.stage_ms().items():
             stage_acc[k] = stage_acc.get(k, 0.0) + v
-    t1 = time.perf_counter()
+    ms_events = (time.perf_counter() - t0) * 1e3 / args.steps
+    ctx.set_timing(False)
     stats = pass_stats(ctx)
-    ms_batch = (t1 - t0) * 1e3 / args.steps
     stages = {k: v / args.steps for k, v in stage_acc.items()}
 
     log("bench: plain path (one mesh per pass)")
@@ -238,6 +261,7 @@ def run_batch_workload(args):
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_batch, 3),
+        "ms_per_step_with_stage_events": round(ms_events, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -267,6 +291,13 @@ def run_batch_workload(args):
     for dm in dms:
         dm.free()
     ctx.close()
+    if args.served_requests > 0:
+        # the metric's latency figures: a short served stream of the same
+        # kind of meshes (Poisson arrivals, continuous batching)
+        out["served"] = served_stream(args.served_requests, args.served_warmup_batch,
+                                      args.max_batch, args.lanes, args.max_wait_us)
+        out["p50_ms"] = out["served"]["p50_ms"]
+        out["p99_ms"] = out["served"]["p99_ms"]
     if not args.no_cpu:
         log("bench: CPU baseline (oracle, bounded sample)")
         out["cpu_baseline"] = cpu_baseline(meshes)
@@ -373,34 +404,35 @@ def run_single_workload(args, name, P, T, res, key_bits, flags=0, desc=""):
     return out
 
 
-def run_served_workload(args):
-    """Poisson request stream admitted under a 64 GB budget (continuous batching).
-    An untimed warm-up stream of the same kind (another seed) runs first, as
-    the other workloads run warm-up steps: the first requests of a process
-    pay for first-time device allocations (reported as cold_p99_ms)."""
+def served_stream(n, n_warm, max_batch, lanes, max_wait_us):
+    """Poisson request stream at 2000 req/s (95% mixed meshes, 5% icosphere-4,
+    res 64-512) admitted under a 64 GB budget (continuous batching), 1000
+    neighbor queries per resident tree.  An untimed warm-up stream of the same
+    kind (another seed) runs first, as the other workloads run warm-up steps:
+    the first requests of a process pay for first-time device allocations
+    (reported as cold_p99_ms).  Latency = completion - submission."""
     import numpy as np
     import svo_amd
     from svo_amd import meshgen
     cache = {}
 
-    def stream(n, seed):
+    def stream(k, seed):
         out = []
-        for t_arr, s in meshgen.request_stream(n, rate=2000.0, seed=seed):
+        for t_arr, s in meshgen.request_stream(k, rate=2000.0, seed=seed):
             key = (s.kind, s.params)
             if key not in cache:
                 cache[key] = s.build()
             out.append((t_arr, cache[key][0], cache[key][1], s.res))
         return out
 
-    n = args.requests
     meshes = stream(n, 4)
-    warm = stream(args.served_warmup, 14) if args.served_warmup > 0 else []
+    warm = stream(n_warm, 14) if n_warm > 0 else []
     ctx = svo_amd.Context(0)
     # warm the lanes (kernel code objects, workspaces)
     P0, T0 = meshgen.icosphere(3)
     ctx.build(P0, T0, 64).free()
-    srv = svo_amd.Server(ctx, mem_budget=64 << 30, max_batch=args.max_batch,
-                         n_lanes=args.lanes, max_wait_us=args.max_wait_us)
+    srv = svo_amd.Server(ctx, mem_budget=64 << 30, max_batch=max_batch, n_lanes=lanes,
+                         max_wait_us=max_wait_us)
     rng = np.random.default_rng(5)
 
     def run(reqs):
@@ -434,27 +466,42 @@ def run_served_workload(args):
         log(f"bench: served warm-up stream of {len(warm)} requests (untimed)")
         wl, _, _, _, _ = run(warm)
         cold_p99 = round(float(np.percentile(np.array(wl) * 1e3, 99)), 3)
+    log(f"bench: served stream of {len(meshes)} requests")
     lat, bsz, wait, queries, wall = run(meshes)
     srv.close()
     ctx.close()
     lat = np.array(lat) * 1e3
     return {
-        "metric": METRIC, "value": round(n / wall, 2), "unit": "meshes/s", "n_gpus": 1,
-        "steps": 1, "warmup": len(warm), "ms_per_step": round(wall * 1e3, 3),
+        "requests": n, "warmup_requests": len(warm), "wall_ms": round(wall * 1e3, 3),
+        "meshes_per_s": round(n / wall, 2),
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "cold_p99_ms": cold_p99,
+        "queue_wait_p50_ms": round(float(np.percentile(np.array(wait) * 1e3, 50)), 3),
+        "queue_wait_p99_ms": round(float(np.percentile(np.array(wait) * 1e3, 99)), 3),
+        "mean_batch": round(float(np.mean(bsz)), 2), "queries": queries,
+        "server": {"lanes": lanes, "max_batch": max_batch, "max_wait_us": max_wait_us,
+                   "budget_gb": 64},
+    }
+
+
+def run_served_workload(args):
+    """The served stream alone: one JSON line whose value is its meshes/s."""
+    r = served_stream(args.requests, args.served_warmup, args.max_batch, args.lanes,
+                      args.max_wait_us)
+    n = r["requests"]
+    return {
+        "metric": METRIC, "value": r["meshes_per_s"], "unit": "meshes/s", "n_gpus": 1,
+        "steps": 1, "warmup": r["warmup_requests"], "ms_per_step": r["wall_ms"],
         "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u32", "data": "synthetic",
         "config": {"workload": f"served: {n} requests, Poisson 2000 req/s, 95% mixed meshes + 5% "
                                f"icosphere-4, res 64-512, 64 GB budget, 1000 neighbor queries "
                                f"per resident tree, after an untimed warm-up stream of "
-                               f"{len(warm)} requests", "parallelism": "single GPU"},
-        "cold_p99_ms": cold_p99,
-        "p50_ms": round(float(np.percentile(lat, 50)), 3),
-        "p99_ms": round(float(np.percentile(lat, 99)), 3),
-        "queue_wait_p50_ms": round(float(np.percentile(np.array(wait) * 1e3, 50)), 3),
-        "queue_wait_p99_ms": round(float(np.percentile(np.array(wait) * 1e3, 99)), 3),
-        "server": {"lanes": args.lanes, "max_batch": args.max_batch,
-                   "max_wait_us": args.max_wait_us},
-        "mean_batch": round(float(np.mean(bsz)), 2), "queries": queries,
+                               f"{r['warmup_requests']} requests", "parallelism": "single GPU"},
+        "cold_p99_ms": r["cold_p99_ms"], "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+        "queue_wait_p50_ms": r["queue_wait_p50_ms"], "queue_wait_p99_ms": r["queue_wait_p99_ms"],
+        "server": r["server"], "mean_batch": r["mean_batch"], "queries": r["queries"],
         "roofline": None, "cpu_baseline": None,
     }
 
@@ -554,6 +601,11 @@ def main():
     ap.add_argument("--requests", type=int, default=2000)
     ap.add_argument("--served-warmup", type=int, default=500,
                     help="served: requests of an untimed warm-up stream run first")
+    ap.add_argument("--served-requests", type=int, default=800,
+                    help="batch workload: requests of the served stream whose p50 / p99 "
+                         "latency go into the line (0: none)")
+    ap.add_argument("--served-warmup-batch", type=int, default=300,
+                    help="batch workload: untimed warm-up requests of that stream")
     ap.add_argument("--max-batch", type=int, default=32)
     ap.add_argument("--max-wait-us", type=int, default=1000)
     ap.add_argument("--lanes", type=int, default=2, help="served: pipeline lanes")

@@ -10,17 +10,23 @@
 // the caller frees them; the budget check sees that through the context's
 // byte counter.
 //
-// A request's bound is the grid bound (every level full, O(1)) when that is
-// at most 1/32 of the budget; otherwise intake threads compute the bound from
-// its triangles (one host pass, estimate_bytes_mesh) while earlier requests
-// build, and the request is admitted once that is known.  Without a budget no
-// bound is needed.
+// What a request reserves: the grid bound (every level full; O(1), an upper
+// bound) when that is at most 1/32 of the budget; else the count-only
+// estimate for a closed surface (O(1), not a bound) when that is below 1/8 of
+// the budget; else the bound from its triangles (estimate_bytes_mesh, a host
+// pass of a few ns per triangle plus the marking of its level-2 cells),
+// which intake threads compute while earlier requests build -- the request
+// is admitted once it is known.  Computing that bound for every request would
+// cost milliseconds of host time per large-resolution request, more than its
+// build, so small requests are admitted on the estimate and a pass that runs
+// out of memory is retried.  Without a budget nothing is reserved.
 //
 // Results are per request: a batch can run as several passes, and when one of
 // them fails the requests of the others still complete with their trees.  A
-// request whose pass ran out of memory anyway (device memory taken by other
-// users of the context) goes back to the head of the queue and is retried
-// while it is less than a second old; after that it completes with SVO_E_OOM.
+// request whose pass ran out of memory (an estimate below the real need, or
+// device memory taken by other users of the context) goes back to the head of
+// the queue and is retried while it is less than a second old; after that it
+// completes with SVO_E_OOM.
 #include <chrono>
 #include <deque>
 #include <thread>
@@ -223,8 +229,19 @@ int svo_server_submit(svo_server* srv, const svo_mesh_desc* mesh, const svo_para
     forced_sort = srv->ctx->tuning.force_path == 2;
   }
   const uint64_t budget = srv->cfg.mem_budget;
-  r->estimate = grid_bound_bytes(params, kb, mesh->n_points, mesh->n_tris);
-  r->ready = !budget || (!forced_sort && r->estimate <= budget / 32);
+  const uint64_t grid = grid_bound_bytes(params, kb, mesh->n_points, mesh->n_tris);
+  const uint64_t quick = estimate_bytes(mesh->n_points, mesh->n_tris, params);
+  r->ready = true;
+  if (!budget) {
+    r->estimate = quick;
+  } else if (!forced_sort && grid <= budget / 32) {
+    r->estimate = grid;
+  } else if (quick < budget / 8) {
+    r->estimate = quick;
+  } else {
+    r->estimate = grid;  // replaced by the mesh bound before admission
+    r->ready = false;
+  }
   r->t_submit = now_s();
   {
     std::lock_guard<std::mutex> lk(srv->mu);
