@@ -244,6 +244,35 @@ def run_batch_workload(args):
     stats = pass_stats(ctx)
     stages = {k: v / args.steps for k, v in stage_acc.items()}
 
+    graph = None
+    if not args.no_graph:
+        # the same batch recorded once and replayed as one HIP graph per step
+        # (svo_plan_*): identical outputs, one host synchronization per step
+        log("bench: batch plan (recording, capture)")
+        plan = ctx.plan(descs, on_device=True)
+        replays = []
+
+        def graph_step():
+            trees, st, replayed = plan.run()
+            if any(s != 0 for s in st):
+                raise RuntimeError(f"plan replay failed: statuses {set(st)}")
+            replays.append(replayed)
+            for t in trees:
+                t.free()
+
+        for _ in range(args.warmup):
+            graph_step()
+        log(f"bench: timing {args.steps} graph replays")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            graph_step()
+        ms_graph = (time.perf_counter() - t0) * 1e3 / args.steps
+        plan.close()
+        graph = {"ms_per_step": round(ms_graph, 3),
+                 "meshes_per_s": round(256 / (ms_graph * 1e-3), 2),
+                 "all_replayed": all(replays),
+                 "vs_eager_batched": round(ms_batch / ms_graph, 3)}
+
     log("bench: plain path (one mesh per pass)")
     plain_step()  # warmup
     ps = max(1, args.plain_steps)
@@ -278,6 +307,7 @@ def run_batch_workload(args):
         "plain_meshes_per_s": round(256 / (ms_plain * 1e-3), 2),
         "plain_ms_per_pass_of_256": round(ms_plain, 3),
         "batched_vs_plain": round(ms_plain / ms_batch, 3),
+        "graph_replay": graph,
         "sizes": dict({"voxels": nv, "nodes": nn, "hits": stats["n_hits"]}, **sizes),
         "passes": ctx.get_option("batch_lanes"),
         "tuning": tuning,
@@ -610,6 +640,8 @@ def main():
     ap.add_argument("--max-wait-us", type=int, default=1000)
     ap.add_argument("--lanes", type=int, default=2, help="served: pipeline lanes")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch workload: skip the batch-plan (HIP graph) leg")
     ap.add_argument("--autotune", action="store_true",
                     help="batch workload: run the library's tuning search on the batch first")
     ap.add_argument("--batch-lanes", type=int, default=0,

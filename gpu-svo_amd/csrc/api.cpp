@@ -308,6 +308,51 @@ int svo_build_batch(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params*
   return agg;
 }
 
+// ---- batch plans (recorded once, replayed as one HIP graph)
+struct svo_plan {
+  std::unique_ptr<BatchPlan> impl;
+};
+
+int svo_plan_create(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params* params,
+                    uint32_t n, int meshes_on_device, svo_plan** out) {
+  if (!ctx || !out || !n || !meshes || !params) return SVO_E_INVALID;
+  *out = nullptr;
+  std::vector<MeshInput> in(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    in[i].desc = meshes[i];
+    in[i].params = params[i];
+    in[i].on_device = meshes_on_device != 0;
+  }
+  auto p = std::make_unique<svo_plan>();
+  const int st = plan_create(*ctx->impl, in, p->impl);
+  if (st != SVO_OK) return st;
+  *out = p.release();
+  return SVO_OK;
+}
+
+int svo_plan_run(svo_plan* plan, svo_tree** trees, int* statuses, int* replayed) {
+  if (!plan || !trees) return SVO_E_INVALID;
+  std::vector<svo_tree*> tr;
+  std::vector<int> st;
+  bool rp = false;
+  const int s = plan_run(*plan->impl, tr, st, rp);
+  if (replayed) *replayed = rp ? 1 : 0;
+  int agg = s;
+  for (size_t i = 0; i < plan->impl->in.size(); ++i) {
+    trees[i] = i < tr.size() ? tr[i] : nullptr;
+    const int si = i < st.size() ? st[i] : s;
+    if (statuses) statuses[i] = si;
+    if (agg == SVO_OK && si != SVO_OK) agg = si;
+  }
+  return agg;
+}
+
+int svo_plan_destroy(svo_plan* plan) {
+  if (!plan) return SVO_E_INVALID;
+  delete plan;
+  return SVO_OK;
+}
+
 // ---- tuning search and persistence
 namespace {
 double now_ms() {

@@ -131,6 +131,7 @@ __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
     const uint32_t* group_next, const uint32_t* first_child_next, uint64_t n_groups,
     const LevelSpan* spans, uint32_t n_levels, const MeshTreeState* state,
     const MeshNodeLayout* layout, const MeshParams* params, NodeArrays out) {
+  if (out.skip && *out.skip) return;  // graph replay over changed sizes
   const uint64_t base = (uint64_t)blockIdx.x * BLOCK * NODES_PT + threadIdx.x;
   const uint32_t k = (uint32_t)(threadIdx.x & 7);
   NodeIn<K> a[NODES_PT];
@@ -216,6 +217,7 @@ __global__ void k_emit_roots(KeyLayout<K> kl, LevelPtrs lp, uint32_t n_meshes,
                              const LevelSpan* spans, uint32_t n_levels,
                              const MeshTreeState* state, const MeshNodeLayout* layout,
                              const MeshParams* params, NodeArrays out) {
+  if (out.skip && *out.skip) return;  // graph replay over changed sizes
   const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n_meshes) return;
   const uint32_t d = state[m].depth;
@@ -288,6 +290,7 @@ __global__ __launch_bounds__(BLOCK) void k_neighbors_level(
     int level, KeyLayout<K> kl, const K* elem_next, const uint32_t* group_next,
     uint64_t n_groups, const LevelSpan* spans, uint32_t n_levels, const MeshTreeState* state,
     const MeshNodeLayout* layout, NodeArrays nodes) {
+  if (nodes.skip && *nodes.skip) return;  // graph replay over changed sizes
   const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
   const uint64_t g = gid >> 3;
   const uint32_t k = (uint32_t)(gid & 7);
@@ -511,6 +514,7 @@ __global__ __launch_bounds__(BLOCK) void k_attr_count(int semantics, NodeArrays 
                                                       uint64_t n_nodes, const uint64_t* mbase,
                                                       uint32_t n_meshes, uint32_t G,
                                                       uint64_t* partial) {
+  if (nodes.skip && *nodes.skip) return;  // graph replay over changed sizes
   __shared__ uint64_t tmp[4];
   uint64_t beg, end;
   block_range(n_nodes, G, blockIdx.x, beg, end);
@@ -561,6 +565,7 @@ __global__ __launch_bounds__(BLOCK) void k_attr_write(int semantics, NodeArrays 
                                                       uint32_t n_meshes, uint32_t G,
                                                       const uint64_t* partial, AttrOut out,
                                                       uint64_t* mesh_first) {
+  if (nodes.skip && *nodes.skip) return;  // graph replay over changed sizes
   __shared__ uint64_t tmp[4];
   uint64_t beg, end;
   block_range(n_nodes, G, blockIdx.x, beg, end);
@@ -742,6 +747,25 @@ void launch_copy_voxels(KeyLayout<K> kl, const void* elem1, uint64_t n, void* ou
   if (n == 0) return;
   hipLaunchKernelGGL(k_copy_voxels<K>, dim3(grid_for(n)), dim3(BLOCK), 0, s, kl,
                      (const K*)elem1, n, out, out_bytes);
+}
+
+// Sets *flag when any of the n words differs from its expected value.
+__global__ void k_check_equal(const uint32_t* got, const uint32_t* want, uint64_t n,
+                              uint32_t* flag) {
+  uint32_t diff = 0;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    diff |= got[i] ^ want[i];
+  if (__any(diff != 0) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+void launch_check_equal(const void* got, const void* want, uint64_t bytes, uint32_t* flag,
+                        hipStream_t s) {
+  const uint64_t n = bytes / 4;
+  if (n == 0) return;
+  const uint64_t blocks = std::min<uint64_t>((n + BLOCK - 1) / BLOCK, 256);
+  hipLaunchKernelGGL(k_check_equal, dim3((uint32_t)blocks), dim3(BLOCK), 0, s,
+                     (const uint32_t*)got, (const uint32_t*)want, n, flag);
 }
 
 void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s) {

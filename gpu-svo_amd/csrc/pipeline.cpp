@@ -204,6 +204,11 @@ int Workspace::get(const char* name, size_t bytes, void** out) {
   // grow when too small; give back a buffer far larger than this pass needs
   // (after one huge pass the lane would otherwise hold its memory for good)
   const bool oversize = b.bytes > 4 * bytes && b.bytes - bytes > (1ull << 30);
+  if (frozen) {
+    if (!b.ptr || b.bytes < bytes) return SVO_E_HIP;  // would allocate
+    *out = b.ptr;
+    return SVO_OK;
+  }
   if (b.bytes < bytes || oversize) {
     if (b.ptr) {
       ctx->dev_free(b.ptr, b.bytes);
@@ -694,9 +699,10 @@ struct StageTimer {
   };
   std::vector<Range> ranges;
   std::vector<hipEvent_t> side_ev;
-  StageTimer(Context& c, Lane& l) : C(c), L(l) {}
+  bool off = false;  // no events (a pass under graph capture)
+  StageTimer(Context& c, Lane& l, bool disabled = false) : C(c), L(l), off(disabled) {}
   void mark(const char* name) {
-    if (!C.timing) return;
+    if (!C.timing || off) return;
     if (L.events.size() <= used) {
       hipEvent_t e;
       if (hipEventCreate(&e) != hipSuccess) return;
@@ -708,7 +714,7 @@ struct StageTimer {
   }
   // Records a timing event on `st` (the side stream); returns its index.
   size_t side_mark(hipStream_t st) {
-    if (!C.timing) return 0;
+    if (!C.timing || off) return 0;
     const size_t i = side_ev.size();
     // side events live in L.events after a fixed block of 64 main marks
     const size_t slot = 64 + i;
@@ -781,17 +787,6 @@ bool no_adjacent_duplicates(const std::vector<MeshNodeLayout>& lay, const MeshPa
   return true;
 }
 
-struct Plan {
-  CallTiming* timing = nullptr;
-  Tuning tuning;              // snapshot taken under the context's mutex
-  std::vector<uint32_t> idx;  // active mesh -> input index
-  int bits = 1;               // per-axis coordinate bits of the batch
-  int cb = 3;                 // code bits
-  int mb = 0;                 // mesh bits
-  bool k64 = false;
-  int n_levels = 2;
-  int code_bytes = 4;
-};
 
 template <typename K>
 int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan& plan,
@@ -800,7 +795,12 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   Workspace& W = L.ws;
   const uint32_t n = (uint32_t)plan.idx.size();
   const uint32_t NL = (uint32_t)plan.n_levels;
-  StageTimer timer(C, L);
+  // A captured pass reads nothing back: the host values come from the
+  // recorded run, the device compares its own with them (rec->d_flag), and
+  // the buffers uploaded by the recorded run are still in place.
+  PassRecord* rec = plan.rec;
+  const bool capture = rec && rec->mode == PassRecord::CAPTURE;
+  StageTimer timer(C, L, capture);
   KeyLayout<K> kl;
   kl.cb = plan.cb;
   kl.cmask = (plan.cb >= (int)(8 * sizeof(K))) ? (K)~(K)0 : (K)(((K)1 << plan.cb) - 1);
@@ -814,6 +814,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     if (!mi.on_device) host_bytes += align256(12 * mi.desc.n_points) + align256(12 * mi.desc.n_tris);
   }
   char* staging = nullptr;
+  if (host_bytes && capture) return SVO_E_INVALID;  // plans hold device inputs
   if (host_bytes) SVO_TRY(W.get("input", host_bytes, (void**)&staging));
   uint64_t soff = 0;
   for (uint32_t i = 0; i < n; ++i) {
@@ -871,8 +872,8 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
       (unsigned long long*)(ctl + sz_md + sz_tb + sz_bo + sz_acc + sz_par);
   // d_cnt: [0] box cells, [1] big chunks, [2] keys emitted, [3] big queued,
   // [4] bitmap bytes zeroed by the level-1 pass
-  SVO_TRY(L.ensure_pinned(sz_md + sz_tb + sz_bo + 64 + sizeof(MeshParams) * n));
-  {
+  if (!capture) {
+    SVO_TRY(L.ensure_pinned(sz_md + sz_tb + sz_bo + 64 + sizeof(MeshParams) * n));
     // stage the mesh descriptors, triangle bases and bitmap offsets through
     // the pinned buffer in one copy; the host does not touch the buffer again
     // before the next stream synchronization
@@ -897,10 +898,21 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
                                                     std::max<uint64_t>(32, T / 128));
   launch_classify(d_md, d_par, d_tb, n, T, tn, d_cnt, s);
   timer.mark("classify");
-  HIP_TRY(hipMemcpyAsync(L.pinned, d_cnt, 16, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  uint64_t cells = ((uint64_t*)L.pinned)[0];
-  const uint64_t n_big = ((uint64_t*)L.pinned)[1];
+  uint64_t cells = 0, n_big = 0;
+  if (capture) {
+    cells = rec->a0[0];
+    n_big = rec->a0[1];
+    launch_check_equal(d_cnt, rec->d_expect, 16, rec->d_flag, s);
+  } else {
+    HIP_TRY(hipMemcpyAsync(L.pinned, d_cnt, 16, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    cells = ((uint64_t*)L.pinned)[0];
+    n_big = ((uint64_t*)L.pinned)[1];
+    if (rec) {
+      rec->a0[0] = cells;
+      rec->a0[1] = n_big;
+    }
+  }
   if (cells >= (1ull << 32) - 1) return SVO_E_OOM;  // key counts are 32-bit in the sort
   const uint64_t key_cap = std::max<uint64_t>(cells, 1);
 
@@ -911,9 +923,17 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
                 bm_bytes <= (uint64_t)plan.tuning.bitmap_bytes_per_cell * key_cap;
   if (plan.tuning.force_path == 1) use_bm = true;
   if (plan.tuning.force_path == 2) use_bm = false;
-  // level-1 capacity: unique voxels <= min(candidate cells, bitmap bits)
-  const uint64_t u_cap = use_bm ? std::max<uint64_t>(1, std::min<uint64_t>(key_cap, 32 * bm_words))
-                                : key_cap;
+  // level-1 capacity: unique voxels <= min(candidate cells, bitmap bits).  A
+  // pass of a plan sizes its levels for every grid cell and takes the bitmap
+  // path (whose writes stay inside the grid), so that a replay over meshes
+  // whose sizes changed writes nothing out of bounds before its checks stop it.
+  if (rec) {
+    if (bm_bytes > plan.tuning.bitmap_max_bytes) return SVO_E_RES_LIMIT;
+    use_bm = true;
+  }
+  const uint64_t u_cap =
+      use_bm ? std::max<uint64_t>(1, rec ? 32 * bm_words : std::min<uint64_t>(key_cap, 32 * bm_words))
+             : key_cap;
 
   // ---- workspace for A1
   K *keys_a = nullptr, *keys_b = nullptr;
@@ -1029,19 +1049,34 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   const size_t off_spans = 8 * (NL + 2), off_state = off_spans + sizeof(LevelSpan) * n * NL,
                off_par = off_state + sizeof(MeshTreeState) * n,
                off_hits = (off_par + sizeof(MeshParams) * n + 7) & ~(size_t)7;
-  SVO_TRY(L.ensure_pinned(off_hits + 24));
-  char* hp = (char*)L.pinned;
+  char* hp = nullptr;
+  if (capture) {
+    if (rec->a1.size() != off_hits + 24) return SVO_E_INVALID;
+    hp = rec->a1.data();
+    const char* e1 = rec->d_expect + 16;
+    launch_check_equal(d_counts, e1, 8 * (NL + 2), rec->d_flag, s);
+    launch_check_equal(d_spans, e1 + off_spans, sizeof(LevelSpan) * n * NL, rec->d_flag, s);
+    launch_check_equal(d_state, e1 + off_state, sizeof(MeshTreeState) * n, rec->d_flag, s);
+    launch_check_equal(d_par, e1 + off_par, sizeof(MeshParams) * n, rec->d_flag, s);
+    launch_check_equal(d_cnt + 2, e1 + off_hits, 24, rec->d_flag, s);
+  } else {
+    SVO_TRY(L.ensure_pinned(off_hits + 24));
+    hp = (char*)L.pinned;
+  }
   uint64_t* h_counts = (uint64_t*)hp;
   LevelSpan* h_spans = (LevelSpan*)(hp + off_spans);
   MeshTreeState* h_state = (MeshTreeState*)(hp + off_state);
   MeshParams* h_par = (MeshParams*)(hp + off_par);
   uint64_t* h_hits = (uint64_t*)(hp + off_hits);
-  HIP_TRY(hipMemcpyAsync(h_counts, d_counts, 8 * (NL + 2), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(h_spans, d_spans, sizeof(LevelSpan) * n * NL, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(h_state, d_state, sizeof(MeshTreeState) * n, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(h_par, d_par, sizeof(MeshParams) * n, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(h_hits, d_cnt + 2, 24, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  if (!capture) {
+    HIP_TRY(hipMemcpyAsync(h_counts, d_counts, 8 * (NL + 2), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_spans, d_spans, sizeof(LevelSpan) * n * NL, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_state, d_state, sizeof(MeshTreeState) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_par, d_par, sizeof(MeshParams) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_hits, d_cnt + 2, 24, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (rec) rec->a1.assign(hp, hp + off_hits + 24);
+  }
   const uint64_t n_hits = h_hits[0], bm_reset = h_hits[2];
   if (n_hits > key_cap) return SVO_E_HIP;  // cannot happen: hits <= box cells
   // the pinned area is reused below: keep host copies of the readback
@@ -1107,14 +1142,20 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   const size_t o_v = take(want_attrs ? 16 * 8 * N : 0);
   const size_t o_e = take(want_attrs ? 28 * 12 * N : 0);
   const size_t o_f = take(want_attrs && sem == SVO_SEM_INTENDED ? 28 * 6 * N : 0);
-  auto out = std::make_shared<BatchOutput>();
-  out->block = std::make_shared<DeviceBlock>();
-  {
+  std::shared_ptr<BatchOutput> out;
+  if (capture) {
+    // every replay writes the recorded run's block
+    if (!rec->out || rec->out->block->bytes != off) return SVO_E_INVALID;
+    out = rec->out;
+  } else {
+    out = std::make_shared<BatchOutput>();
+    out->block = std::make_shared<DeviceBlock>();
     void* p = nullptr;
     SVO_TRY(C.dev_malloc(&p, off));
     out->block->ctx = C.shared_from_this();
     out->block->ptr = p;
     out->block->bytes = off;
+    if (rec) rec->out = out;
   }
   // From here on kernels of this pass write the output block.  On an error
   // return, wait for both streams before `out` (declared above, so destroyed
@@ -1140,6 +1181,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   na.childs = (uint32_t*)(ob + o_ch);
   na.neighbors = (uint32_t*)(ob + o_nb);
   na.code_bytes = cbytes;
+  na.skip = capture ? rec->d_flag : nullptr;
   out->voxels = ob + o_vx;
   out->verts = (svo_vertex*)(ob + o_v);
   out->edges = (svo_edge*)(ob + o_e);
@@ -1154,10 +1196,12 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   MeshNodeLayout* d_lay = (MeshNodeLayout*)bctl;
   uint64_t* d_nbase = (uint64_t*)(bctl + sz_lay);
   unsigned long long* d_tot = (unsigned long long*)(bctl + sz_lay + sz_nb);
-  SVO_TRY(L.ensure_pinned(sz_lay + sz_nb + sz_tot));
-  std::memcpy(L.pinned, lay.data(), sizeof(MeshNodeLayout) * n);
-  std::memcpy((char*)L.pinned + sz_lay, node_base.data(), 8 * (n + 1));
-  HIP_TRY(hipMemcpyAsync(bctl, L.pinned, sz_lay + sz_nb, hipMemcpyHostToDevice, s));
+  if (!capture) {
+    SVO_TRY(L.ensure_pinned(sz_lay + sz_nb + sz_tot));
+    std::memcpy(L.pinned, lay.data(), sizeof(MeshNodeLayout) * n);
+    std::memcpy((char*)L.pinned + sz_lay, node_base.data(), 8 * (n + 1));
+    HIP_TRY(hipMemcpyAsync(bctl, L.pinned, sz_lay + sz_nb, hipMemcpyHostToDevice, s));
+  }
   launch_fill_u32((uint32_t*)d_tot, 0u, sz_tot / 4, s);
   timer.mark("setup_b");
   // When this pass runs alone on the context, the neighbor fill (a pure
@@ -1165,7 +1209,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   // their loads' latency); the roots' neighbor slot is written after the
   // join.  Concurrent passes already overlap each other, and their extra
   // streams would outnumber the hardware queues, so they fill in order.
-  bool side_fill = C.active_passes.load() == 1;
+  bool side_fill = C.active_passes.load() == 1 && !capture;
   if (side_fill && (hipEventRecord(L.fork, s) != hipSuccess ||
                     hipStreamWaitEvent(L.side, L.fork, 0) != hipSuccess)) {
     hipGetLastError();
@@ -1220,14 +1264,23 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     }
   }
   timer.mark("attr_write");
-  uint64_t* h_tot = (uint64_t*)L.pinned;
-  HIP_TRY(hipMemcpyAsync(h_tot, d_tot, 8 * (4 + 3 * n), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
-  HIP_TRY(hipGetLastError());
+  std::vector<uint64_t> tot;
+  if (capture) {
+    if (rec->tot.size() != 4 + 3 * (size_t)n) return SVO_E_INVALID;
+    tot = rec->tot;
+    launch_check_equal(d_tot, rec->d_expect + 16 + rec->a1.size(), 8 * (4 + 3 * n), rec->d_flag,
+                       s);
+  } else {
+    uint64_t* h_tot = (uint64_t*)L.pinned;
+    HIP_TRY(hipMemcpyAsync(h_tot, d_tot, 8 * (4 + 3 * n), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    HIP_TRY(hipGetLastError());
+    tot.assign(h_tot, h_tot + 4 + 3 * n);
+    if (rec) rec->tot = tot;
+  }
 
   // ---- trees
-  std::vector<uint64_t> tot(h_tot, h_tot + 4 + 3 * n);
-  {
+  if (!capture) {
     svo_pass_stats ps{};
     ps.n_meshes = n;
     ps.n_tris = T;
@@ -1254,7 +1307,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     const uint32_t src = plan.idx[i];
     statuses[src] = mstat[i];
     if (mstat[i] != SVO_OK) continue;
-    svo_tree* t = new svo_tree();
+    svo_tree* t = capture ? &rec->templ[src] : new svo_tree();
     t->out = out;
     t->ctx = C.shared_from_this();
     t->node_base = node_base[i];
@@ -1301,7 +1354,12 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     }
     I.n_hits = n == 1 ? n_hits : 0;
     I.device_bytes = out->block->bytes;
-    trees[src] = t;
+    if (!capture) trees[src] = t;
+  }
+  if (capture) {
+    for (uint32_t i = 0; i < n; ++i) rec->templ_status[plan.idx[i]] = mstat[i];
+    drain.armed = false;  // nothing ran: the graph is being recorded
+    return SVO_OK;
   }
   drain.armed = false;  // both streams were synchronized above
   return SVO_OK;
@@ -1311,16 +1369,10 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
 
 namespace {
 
-// One pipeline pass over the valid meshes `sel` of `in` on one lane.
-int run_pass(Context& C, const std::vector<MeshInput>& in, const std::vector<uint32_t>& sel,
-             std::vector<svo_tree*>& trees, std::vector<int>& statuses, CallTiming& ct) {
-  struct ActivePass {
-    Context& c;
-    explicit ActivePass(Context& x) : c(x) { c.active_passes.fetch_add(1); }
-    ~ActivePass() { c.active_passes.fetch_sub(1); }
-  } active(C);
-  Plan plan;
-  plan.timing = &ct;
+// The pass plan of meshes `sel` of `in`: key layout, levels, tuning snapshot.
+// SVO_E_RES_LIMIT when their composite keys do not fit 64 bits.
+int make_plan(Context& C, const std::vector<MeshInput>& in, const std::vector<uint32_t>& sel,
+              Plan& plan) {
   bool any64 = false;
   for (uint32_t i : sel) {
     plan.idx.push_back(i);
@@ -1329,25 +1381,42 @@ int run_pass(Context& C, const std::vector<MeshInput>& in, const std::vector<uin
               std::max({in[i].params.res[0], in[i].params.res[1], in[i].params.res[2]}) > 1024);
     for (int a = 0; a < 3; ++a) plan.bits = std::max(plan.bits, bits_for(in[i].params.res[a]));
   }
-  if (plan.idx.empty()) return SVO_OK;
   plan.cb = 3 * plan.bits;
   plan.mb = bits_for((int32_t)plan.idx.size());
   plan.n_levels = plan.bits + 1;
-  if (plan.cb + plan.mb > 64) {
-    for (uint32_t i : plan.idx) statuses[i] = SVO_E_RES_LIMIT;
-    return SVO_E_RES_LIMIT;
-  }
+  if (plan.cb + plan.mb > 64) return SVO_E_RES_LIMIT;
   plan.k64 = plan.cb + plan.mb > 32;
   plan.code_bytes = any64 ? 8 : 4;
-  {
-    std::lock_guard<std::mutex> lk(C.mu);
-    plan.tuning = C.tuning;
+  std::lock_guard<std::mutex> lk(C.mu);
+  plan.tuning = C.tuning;
+  return SVO_OK;
+}
+
+int run_plan_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan& plan,
+                   std::vector<svo_tree*>& trees, std::vector<int>& statuses) {
+  return plan.k64 ? run_typed<uint64_t>(C, L, in, plan, trees, statuses)
+                  : run_typed<uint32_t>(C, L, in, plan, trees, statuses);
+}
+
+// One pipeline pass over the valid meshes `sel` of `in` on one lane.
+int run_pass(Context& C, const std::vector<MeshInput>& in, const std::vector<uint32_t>& sel,
+             std::vector<svo_tree*>& trees, std::vector<int>& statuses, CallTiming& ct) {
+  struct ActivePass {
+    Context& c;
+    explicit ActivePass(Context& x) : c(x) { c.active_passes.fetch_add(1); }
+    ~ActivePass() { c.active_passes.fetch_sub(1); }
+  } active(C);
+  if (sel.empty()) return SVO_OK;
+  Plan plan;
+  plan.timing = &ct;
+  if (make_plan(C, in, sel, plan) != SVO_OK) {
+    for (uint32_t i : plan.idx) statuses[i] = SVO_E_RES_LIMIT;
+    return SVO_E_RES_LIMIT;
   }
   Lane* L = C.acquire_lane();
   if (!L) return SVO_E_HIP;
   hipSetDevice(C.device);
-  int st = plan.k64 ? run_typed<uint64_t>(C, *L, in, plan, trees, statuses)
-                    : run_typed<uint32_t>(C, *L, in, plan, trees, statuses);
+  int st = run_plan_typed(C, *L, in, plan, trees, statuses);
   if (st != SVO_OK) {
     // the lane's workspaces are reused by its next pass: let this one end
     hipStreamSynchronize(L->stream);
@@ -1363,15 +1432,12 @@ int run_pass(Context& C, const std::vector<MeshInput>& in, const std::vector<uin
   return st;
 }
 
-// Meshes `valid` (one semantics and flag set) as one pass, or as concurrent
-// sub-batch passes on separate lanes when there are enough of them.
-int run_set(Context& C, const std::vector<MeshInput>& in, const std::vector<uint32_t>& valid,
-            std::vector<svo_tree*>& trees, std::vector<int>& statuses, CallTiming& ct) {
-  // Large batches are split into sub-batches that run as concurrent passes on
-  // separate lanes (streams): their host round trips and kernel tails
-  // overlap.
-  // Meshes go to the lightest sub-batch, heaviest first; the weight estimates
-  // triangles plus surface voxels (~ res^2).
+// Meshes `valid` split into the sub-batches of concurrent passes: large
+// batches are split so that the passes' host round trips and kernel tails
+// overlap.  Meshes go to the lightest sub-batch, heaviest first; the weight
+// estimates triangles plus surface voxels (~ res^2).
+std::vector<std::vector<uint32_t>> split_batch(Context& C, const std::vector<MeshInput>& in,
+                                               const std::vector<uint32_t>& valid) {
   uint32_t groups = 1;
   {
     Tuning t;
@@ -1382,7 +1448,7 @@ int run_set(Context& C, const std::vector<MeshInput>& in, const std::vector<uint
     if (t.batch_lanes > 1 && valid.size() >= 2 * (size_t)t.batch_min_meshes)
       groups = std::min<uint32_t>(t.batch_lanes, (uint32_t)(valid.size() / t.batch_min_meshes));
   }
-  if (groups <= 1) return run_pass(C, in, valid, trees, statuses, ct);
+  if (groups <= 1) return {valid};
   std::vector<std::pair<double, uint32_t>> w;
   for (uint32_t i : valid) {
     const double r = std::max({in[i].params.res[0], in[i].params.res[1], in[i].params.res[2]});
@@ -1397,6 +1463,16 @@ int run_set(Context& C, const std::vector<MeshInput>& in, const std::vector<uint
     load[g] += e.first;
   }
   for (auto& v : sel) std::sort(v.begin(), v.end());
+  return sel;
+}
+
+// Meshes `valid` (one semantics and flag set) as one pass, or as concurrent
+// sub-batch passes on separate lanes when there are enough of them.
+int run_set(Context& C, const std::vector<MeshInput>& in, const std::vector<uint32_t>& valid,
+            std::vector<svo_tree*>& trees, std::vector<int>& statuses, CallTiming& ct) {
+  const std::vector<std::vector<uint32_t>> sel = split_batch(C, in, valid);
+  const uint32_t groups = (uint32_t)sel.size();
+  if (groups <= 1) return run_pass(C, in, valid, trees, statuses, ct);
   std::vector<int> rc(groups, SVO_OK);
   std::vector<std::thread> th;
   for (uint32_t g = 1; g < groups; ++g)
@@ -1409,6 +1485,202 @@ int run_set(Context& C, const std::vector<MeshInput>& in, const std::vector<uint
 }
 
 }  // namespace
+
+// ------------------------------------------------------------------ batch plans
+// A batch recorded once and replayed as one HIP graph: an eager run of each
+// sub-batch pass on a lane of the plan's own records the host values of its
+// three readbacks (PassRecord) and sizes the lane's workspaces; the same
+// passes are then captured -- the sub-batches as parallel branches forked
+// from one stream -- with the readbacks replaced by device-side comparisons
+// against the recorded values.  A replay is one graph launch and one stream
+// synchronization; it recomputes every output from the (device-resident)
+// inputs into the recorded output blocks, and a set flag (a count, span or
+// total that differs from the recorded run) makes svo_plan_run fall back to
+// an eager build.  Replaces the per-level host synchronizations of the
+// reference's level loop (SVO.cu:383-495) by one per batch.
+BatchPlan::~BatchPlan() {
+  Context& C = *ctx;
+  hipSetDevice(C.device);
+  for (auto& l : lanes)
+    if (l->stream) hipStreamSynchronize(l->stream);
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
+  if (fork) hipEventDestroy(fork);
+  for (hipEvent_t e : joins)
+    if (e) hipEventDestroy(e);
+  for (auto& e : expect) C.dev_free(e.first, e.second);
+  if (d_flag) C.dev_free(d_flag, 256);
+  if (h_flag) hipHostFree(h_flag);
+  recs.clear();  // output blocks go back to the context
+  for (auto& l : lanes) {
+    l->ws.release_all();
+    for (auto ev : l->events) hipEventDestroy(ev);
+    if (l->fork) hipEventDestroy(l->fork);
+    if (l->join) hipEventDestroy(l->join);
+    if (l->pinned) hipHostFree(l->pinned);
+    if (l->stream) hipStreamDestroy(l->stream);
+    if (l->side) hipStreamDestroy(l->side);
+  }
+  for (auto& o : owned) C.dev_free(o.first, o.second);
+}
+
+int plan_create(Context& C, const std::vector<MeshInput>& user, std::unique_ptr<BatchPlan>& out) {
+  auto bp = std::make_unique<BatchPlan>();
+  bp->ctx = C.shared_from_this();
+  bp->in = user;
+  HIP_TRY(hipSetDevice(C.device));
+  const uint32_t n = (uint32_t)user.size();
+  if (n == 0) return SVO_E_INVALID;
+  for (uint32_t i = 0; i < n; ++i) {
+    int kb = 32;
+    SVO_TRY(validate_params(&user[i].desc, &user[i].params, &kb));
+    if (user[i].params.semantics != user[0].params.semantics ||
+        user[i].params.flags != user[0].params.flags)
+      return SVO_E_INVALID;  // one semantics and flag set per plan
+  }
+  // lanes of the plan's own: nothing else may touch the workspaces the graph
+  // points into
+  std::vector<uint32_t> all(n);
+  for (uint32_t i = 0; i < n; ++i) all[i] = i;
+  const auto sel = split_batch(C, bp->in, all);
+  if (sel.size() > 64) return SVO_E_INVALID;
+  for (size_t g = 0; g < sel.size(); ++g) {
+    auto l = std::make_unique<Lane>();
+    if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&l->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&l->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&l->join, hipEventDisableTiming) != hipSuccess)
+      return SVO_E_HIP;
+    l->ws.ctx = &C;
+    bp->lanes.push_back(std::move(l));
+  }
+  hipStream_t s0 = bp->lanes[0]->stream;
+  // host meshes become device copies owned by the plan
+  for (uint32_t i = 0; i < n; ++i) {
+    MeshInput& mi = bp->in[i];
+    if (mi.on_device) continue;
+    void* p = nullptr;
+    void* t = nullptr;
+    SVO_TRY(C.dev_malloc(&p, 12 * mi.desc.n_points));
+    bp->owned.push_back({p, 12 * mi.desc.n_points});
+    SVO_TRY(C.dev_malloc(&t, 12 * mi.desc.n_tris));
+    bp->owned.push_back({t, 12 * mi.desc.n_tris});
+    HIP_TRY(hipMemcpyAsync(p, mi.desc.xyz, 12 * mi.desc.n_points, hipMemcpyHostToDevice, s0));
+    HIP_TRY(hipMemcpyAsync(t, mi.desc.tri, 12 * mi.desc.n_tris, hipMemcpyHostToDevice, s0));
+    HIP_TRY(hipStreamSynchronize(s0));
+    mi.desc.xyz = (const float*)p;
+    mi.desc.tri = (const int32_t*)t;
+    mi.on_device = true;
+  }
+  // record: one eager run of every pass
+  CallTiming ct;
+  for (size_t g = 0; g < sel.size(); ++g) {
+    Plan pl;
+    pl.timing = &ct;
+    SVO_TRY(make_plan(C, bp->in, sel[g], pl));
+    auto rec = std::make_unique<PassRecord>();
+    pl.rec = rec.get();
+    std::vector<svo_tree*> trees(n, nullptr);
+    std::vector<int> st(n, SVO_OK);
+    const int rc = run_plan_typed(C, *bp->lanes[g], bp->in, pl, trees, st);
+    for (svo_tree* t : trees) delete t;  // the record keeps the output block
+    if (rc != SVO_OK) return rc;
+    bp->plans.push_back(pl);
+    bp->recs.push_back(std::move(rec));
+  }
+  // expected readbacks on the device, [a0 | a1 | tot] per pass
+  for (auto& rec : bp->recs) {
+    std::vector<char> img(16 + rec->a1.size() + 8 * rec->tot.size());
+    std::memcpy(img.data(), rec->a0, 16);
+    std::memcpy(img.data() + 16, rec->a1.data(), rec->a1.size());
+    std::memcpy(img.data() + 16 + rec->a1.size(), rec->tot.data(), 8 * rec->tot.size());
+    void* d = nullptr;
+    SVO_TRY(C.dev_malloc(&d, img.size()));
+    bp->expect.push_back({d, img.size()});
+    HIP_TRY(hipMemcpyAsync(d, img.data(), img.size(), hipMemcpyHostToDevice, s0));
+    HIP_TRY(hipStreamSynchronize(s0));
+    rec->d_expect = (const char*)d;
+  }
+  SVO_TRY(C.dev_malloc((void**)&bp->d_flag, 256));
+  HIP_TRY(hipHostMalloc((void**)&bp->h_flag, 256));
+  HIP_TRY(hipEventCreateWithFlags(&bp->fork, hipEventDisableTiming));
+  bp->joins.assign(sel.size(), nullptr);
+  for (size_t g = 1; g < sel.size(); ++g)
+    HIP_TRY(hipEventCreateWithFlags(&bp->joins[g], hipEventDisableTiming));
+  for (size_t g = 0; g < sel.size(); ++g) {
+    bp->lanes[g]->ws.frozen = true;
+    PassRecord& r = *bp->recs[g];
+    r.mode = PassRecord::CAPTURE;
+    // a flag per pass: set only by the pass's own checks, which its later
+    // kernels follow in stream order
+    r.d_flag = bp->d_flag + g;
+    r.templ.assign(n, svo_tree());
+    r.templ_status.assign(n, SVO_E_INVALID);
+  }
+  bp->sel = sel;
+  // capture: the passes as branches forked from lane 0's stream
+  HIP_TRY(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+  int rc = SVO_OK;
+  launch_fill_u32(bp->d_flag, 0u, 64, s0);
+  if (hipEventRecord(bp->fork, s0) != hipSuccess) rc = SVO_E_HIP;
+  for (size_t g = 1; g < sel.size() && rc == SVO_OK; ++g)
+    if (hipStreamWaitEvent(bp->lanes[g]->stream, bp->fork, 0) != hipSuccess) rc = SVO_E_HIP;
+  for (size_t g = 0; g < sel.size() && rc == SVO_OK; ++g) {
+    std::vector<svo_tree*> trees(n, nullptr);
+    std::vector<int> st(n, SVO_OK);
+    rc = run_plan_typed(C, *bp->lanes[g], bp->in, bp->plans[g], trees, st);
+  }
+  for (size_t g = 1; g < sel.size() && rc == SVO_OK; ++g)
+    if (hipEventRecord(bp->joins[g], bp->lanes[g]->stream) != hipSuccess ||
+        hipStreamWaitEvent(s0, bp->joins[g], 0) != hipSuccess)
+      rc = SVO_E_HIP;
+  if (rc == SVO_OK &&
+      hipMemcpyAsync(bp->h_flag, bp->d_flag, 4 * sel.size(), hipMemcpyDeviceToHost, s0) !=
+          hipSuccess)
+    rc = SVO_E_HIP;
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s0, &graph);
+  bp->graph = graph;
+  if (rc != SVO_OK) {
+    hipGetLastError();
+    return rc;
+  }
+  if (ec != hipSuccess || !graph) return SVO_E_HIP;
+  HIP_TRY(hipGraphInstantiate(&bp->exec, graph, nullptr, nullptr, 0));
+  out = std::move(bp);
+  return SVO_OK;
+}
+
+int plan_run(BatchPlan& bp, std::vector<svo_tree*>& trees, std::vector<int>& statuses,
+             bool& replayed) {
+  std::lock_guard<std::mutex> lk(bp.mu);
+  Context& C = *bp.ctx;
+  const uint32_t n = (uint32_t)bp.in.size();
+  HIP_TRY(hipSetDevice(C.device));
+  hipStream_t s0 = bp.lanes[0]->stream;
+  const size_t np = bp.sel.size();
+  for (size_t g = 0; g < np; ++g) bp.h_flag[g] = 0xFFFFFFFFu;  // overwritten by the graph
+  HIP_TRY(hipGraphLaunch(bp.exec, s0));
+  HIP_TRY(hipStreamSynchronize(s0));
+  uint32_t any = 0;
+  for (size_t g = 0; g < np; ++g) any |= bp.h_flag[g];
+  if (any != 0) {
+    // something the graph was recorded with changed: build eagerly
+    replayed = false;
+    return run_batch(C, bp.in, trees, statuses);
+  }
+  replayed = true;
+  trees.assign(n, nullptr);
+  statuses.assign(n, SVO_OK);
+  for (size_t g = 0; g < bp.sel.size(); ++g) {
+    const PassRecord& r = *bp.recs[g];
+    for (uint32_t i : bp.sel[g]) {
+      statuses[i] = r.templ_status[i];
+      if (r.templ_status[i] == SVO_OK) trees[i] = new svo_tree(r.templ[i]);
+    }
+  }
+  return SVO_OK;
+}
 
 int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree*>& trees,
               std::vector<int>& statuses) {

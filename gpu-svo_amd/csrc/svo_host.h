@@ -27,6 +27,10 @@ struct Workspace {
   };
   Context* ctx = nullptr;
   std::unordered_map<std::string, Buf> bufs;
+  // A frozen workspace hands out its buffers as they are and allocates
+  // nothing (a pass captured into a graph): a buffer missing or too small is
+  // an error.
+  bool frozen = false;
   int get(const char* name, size_t bytes, void** out);
   // Leading bytes of buffer `name` known to be zero; 0 after (re)allocation.
   size_t& clean_bytes(const char* name) { return bufs[name].clean; }
@@ -135,6 +139,27 @@ struct svo_tree {
 };
 
 namespace svo {
+// The host values one pass reads back at its three synchronization points,
+// kept by a batch plan (api: svo_plan_*): recorded from an eager run, they
+// let the same pass be captured into a HIP graph with no readback at all;
+// the captured pass compares what it computes with them on the device and
+// sets a flag when anything differs.
+struct PassRecord {
+  enum Mode { RECORD, CAPTURE } mode = RECORD;
+  uint64_t a0[2] = {0, 0};  // candidate cells, big-triangle work items
+  std::vector<char> a1;     // level readback: counts, spans, states, params, hits
+  std::vector<uint64_t> tot;  // record totals and per-mesh record offsets
+  std::shared_ptr<BatchOutput> out;  // output block, written by every replay
+  // CAPTURE: expected readbacks on the device ([a0 | a1 | tot]) and the flag
+  const char* d_expect = nullptr;
+  uint32_t* d_flag = nullptr;
+  // CAPTURE: the trees a replay hands out (copied per replay), by input index
+  std::vector<svo_tree> templ;
+  std::vector<int> templ_status;
+};
+}  // namespace svo
+
+namespace svo {
 
 struct MeshInput {
   svo_mesh_desc desc;
@@ -147,6 +172,44 @@ struct MeshInput {
 // failed (HIP error or out of memory).
 int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree*>& trees,
               std::vector<int>& statuses);
+
+// One pipeline pass: which meshes, their key layout and the tuning snapshot.
+struct CallTiming;
+struct Plan {
+  CallTiming* timing = nullptr;
+  Tuning tuning;              // snapshot taken under the context's mutex
+  std::vector<uint32_t> idx;  // active mesh -> input index
+  int bits = 1;               // per-axis coordinate bits of the batch
+  int cb = 3;                 // code bits
+  int mb = 0;                 // mesh bits
+  bool k64 = false;
+  int n_levels = 2;
+  int code_bytes = 4;
+  PassRecord* rec = nullptr;  // set by a batch plan (record or capture)
+};
+
+// A recorded batch replayed as one HIP graph (pipeline.cpp).
+struct BatchPlan {
+  std::shared_ptr<Context> ctx;
+  std::vector<MeshInput> in;  // device-resident inputs
+  std::vector<std::pair<void*, size_t>> owned;   // device copies of host inputs
+  std::vector<std::unique_ptr<Lane>> lanes;      // one per sub-batch pass
+  std::vector<std::vector<uint32_t>> sel;        // meshes of each pass
+  std::vector<Plan> plans;
+  std::vector<std::unique_ptr<PassRecord>> recs;
+  std::vector<std::pair<void*, size_t>> expect;  // expected readbacks on the device
+  uint32_t* d_flag = nullptr;
+  uint32_t* h_flag = nullptr;  // pinned
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::mutex mu;
+  ~BatchPlan();
+};
+int plan_create(Context& C, const std::vector<MeshInput>& in, std::unique_ptr<BatchPlan>& out);
+int plan_run(BatchPlan& bp, std::vector<svo_tree*>& trees, std::vector<int>& statuses,
+             bool& replayed);
 
 int validate_params(const svo_mesh_desc* mesh, const svo_params* p, int* key_bits_out);
 uint64_t estimate_bytes(uint64_t n_points, uint64_t n_tris, const svo_params* p);

@@ -121,6 +121,10 @@ struct NodeArrays {
   uint32_t* childs;      // 8 per node
   uint32_t* neighbors;   // 27 per node
   int code_bytes;
+  // Graph-captured passes: when *skip is nonzero (a size differs from the
+  // recorded run) the node, neighbor and record kernels return at once, so
+  // that nothing is written past the recorded layout.  Null otherwise.
+  const uint32_t* skip;
 };
 
 template <typename K>
@@ -139,6 +143,10 @@ template <typename K>
 void launch_copy_voxels(KeyLayout<K> kl, const void* elem1, uint64_t n, void* out,
                         int out_bytes, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint32_t value, uint64_t n, hipStream_t s);
+// Sets *flag (atomic OR of 1) when the `bytes` bytes (a multiple of 4, both
+// 4-byte aligned) at `got` differ from those at `want`.
+void launch_check_equal(const void* got, const void* want, uint64_t bytes, uint32_t* flag,
+                        hipStream_t s);
 template <typename K>
 void launch_neighbors_level(int level, KeyLayout<K> kl, const void* elem_next,
                             const uint32_t* elem_group_next, uint64_t n_groups,

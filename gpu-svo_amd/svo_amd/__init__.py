@@ -100,6 +100,11 @@ _SIGS = {
                                         C.POINTER(C.c_char_p), C.c_int,
                                         C.POINTER(C.c_int)]),
     "svo_ctx_bytes_in_use": (C.c_uint64, [C.c_void_p]),
+    "svo_plan_create": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(Params), C.c_uint32,
+                                  C.c_int, C.POINTER(C.c_void_p)]),
+    "svo_plan_run": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_int),
+                               C.POINTER(C.c_int)]),
+    "svo_plan_destroy": (C.c_int, [C.c_void_p]),
     "svo_ctx_workspace_bytes": (C.c_uint64, [C.c_void_p]),
     "svo_device_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "svo_device_free": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
@@ -481,6 +486,12 @@ class Context:
         trees = [Tree(out[i], self) if st[i] == OK and out[i] else None for i in range(n)]
         return trees, [st[i] for i in range(n)]
 
+    def plan(self, meshes, semantics=SEM_REFERENCE, key_bits=0, flags=0,
+             on_device=False) -> "BatchPlan":
+        """A batch recorded once and replayed as one HIP graph (svo_plan_*);
+        meshes as for build_batch."""
+        return BatchPlan(self, meshes, semantics, key_bits, flags, on_device)
+
     def autotune(self, meshes, semantics=SEM_REFERENCE, key_bits=0, flags=0, on_device=False,
                  reps=3) -> dict:
         """svo_ctx_autotune on a sample batch (same forms as build_batch); the
@@ -497,6 +508,40 @@ class Context:
 
     def load_tuning(self, path: str):
         _check(lib().svo_ctx_load_tuning(self._h, path.encode()), "load tuning")
+
+
+class BatchPlan:
+    """svo_plan_create / svo_plan_run / svo_plan_destroy.  run() returns
+    (trees, statuses, replayed); the trees stay valid until the next run()."""
+
+    def __init__(self, ctx: Context, meshes, semantics, key_bits, flags, on_device):
+        self.ctx = ctx
+        self.n = len(meshes)
+        descs, params, keep = ctx._batch_args(meshes, semantics, key_bits, flags, on_device)
+        h = C.c_void_p()
+        _check(lib().svo_plan_create(ctx.handle, descs, params, self.n, 1 if on_device else 0,
+                                     C.byref(h)), "plan")
+        self._h = h
+
+    def run(self):
+        out = (C.c_void_p * self.n)()
+        st = (C.c_int * self.n)()
+        rp = C.c_int()
+        lib().svo_plan_run(self._h, out, st, C.byref(rp))
+        trees = [Tree(out[i], self.ctx) if st[i] == OK and out[i] else None
+                 for i in range(self.n)]
+        return trees, [st[i] for i in range(self.n)], bool(rp.value)
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().svo_plan_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class DeviceMesh:

@@ -154,6 +154,26 @@ int svo_ctx_destroy(svo_ctx* ctx);
  * stage, the window from the stage's earliest start to its latest end. */
 int svo_ctx_set_timing(svo_ctx* ctx, int enable);
 int svo_ctx_last_stage_ms(svo_ctx* ctx, double* ms, const char** names, int cap, int* n);
+/* Batch plans (no counterpart in the reference; they replace the per-level
+ * host synchronizations of its level loop, SVO.cu:383-495, by one per
+ * batch).  svo_plan_create builds the batch once eagerly, recording the
+ * sizes every pass reads back, then captures the whole batched build -- the
+ * sub-batch passes as parallel branches -- in one HIP graph.  Host meshes are
+ * copied to device memory the plan owns; device meshes must stay allocated
+ * while the plan lives.  All meshes of a plan share one semantics and flag
+ * set.  svo_plan_run replays the graph (one launch, one synchronization),
+ * recomputing every output from the current mesh contents; when a count,
+ * span or total differs from the recorded build (the mesh contents changed
+ * their sizes) it builds eagerly instead and sets *replayed to 0.  The trees
+ * of a replay share the plan's output blocks: they stay valid until the
+ * next svo_plan_run or svo_plan_destroy (free them with svo_tree_free).
+ * Calls on one plan must not overlap. */
+typedef struct svo_plan svo_plan;
+int svo_plan_create(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params* params,
+                    uint32_t n, int meshes_on_device, svo_plan** out);
+int svo_plan_run(svo_plan* plan, svo_tree** trees, int* statuses, int* replayed);
+int svo_plan_destroy(svo_plan* plan);
+
 /* Tuning options of the context (no counterpart in the reference), set
  * between builds:
  *   "small_max_cells"        box cells up to which one lane voxelizes a triangle (1024)
