@@ -175,6 +175,29 @@ def cpu_baseline(meshes, budget_s=12.0):
                       f"on {nt} threads (one mesh per thread)"}
 
 
+TUNING_FILE = os.path.join(ROOT, "gpu-svo_amd", "tuning", "mi355x.tuning")
+
+
+def load_tuning(ctx, args):
+    """Loads the tuning file (the library's tuning search, run on this kind of
+    device by gpu-svo_amd/tuning/make_tuning.py) onto the context; returns
+    what it set, for the JSON line."""
+    path = args.tuning
+    if path == "none" or not os.path.exists(path):
+        return None
+    defaults = {}
+    names = []
+    with open(path) as f:
+        for line in f:
+            if line.strip():
+                names.append(line.split()[0])
+    for nm in names:
+        defaults[nm] = ctx.get_option(nm)
+    ctx.load_tuning(path)
+    changed = {nm: ctx.get_option(nm) for nm in names if ctx.get_option(nm) != defaults[nm]}
+    return {"file": os.path.relpath(path, ROOT), "options": changed}
+
+
 def run_batch_workload(args):
     import numpy as np
     import svo_amd
@@ -191,10 +214,10 @@ def run_batch_workload(args):
         ctx.set_option(name, int(value))
     dms = [ctx.upload_mesh(P, T) for P, T, _ in meshes]
     descs = [dm.desc(res) for dm, (_, _, res) in zip(dms, meshes)]
-    tuning = None
+    tuning = load_tuning(ctx, args)
     if args.autotune:
         log("bench: tuning search on the batch (untimed)")
-        tuning = ctx.autotune(descs, on_device=True, reps=5)
+        tuning = {"search": ctx.autotune(descs, on_device=True, reps=5)}
 
     sizes = {}
 
@@ -642,6 +665,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="batch workload: skip the batch-plan (HIP graph) leg")
+    ap.add_argument("--tuning", default=TUNING_FILE,
+                    help="tuning file loaded on the batch context ('none': library defaults)")
     ap.add_argument("--autotune", action="store_true",
                     help="batch workload: run the library's tuning search on the batch first")
     ap.add_argument("--batch-lanes", type=int, default=0,

@@ -130,7 +130,24 @@ const OptionRef kOptions[] = {
     {"dedup_path", 0, 2},
     {"batch_lanes", 1, 8},
     {"batch_min_meshes", 1, 1 << 20},
+    {"class_small_max_0", 0, 1 << 20},  {"class_small_max_1", 0, 1 << 20},
+    {"class_small_max_2", 0, 1 << 20},  {"class_small_max_3", 0, 1 << 20},
+    {"class_small_max_4", 0, 1 << 20},  {"class_small_max_5", 0, 1 << 20},
+    {"class_small_max_6", 0, 1 << 20},  {"class_small_max_7", 0, 1 << 20},
+    {"class_small_max_8", 0, 1 << 20},  {"class_small_max_9", 0, 1 << 20},
+    {"class_small_max_10", 0, 1 << 20}, {"class_small_max_11", 0, 1 << 20},
+    {"class_small_max_12", 0, 1 << 20}, {"class_small_max_13", 0, 1 << 20},
+    {"class_small_max_14", 0, 1 << 20},
 };
+static_assert(SIZE_CLASSES == 15, "one class_small_max option per size class");
+
+// class index of a "class_small_max_<k>" name, or -1
+int class_option(const std::string& n) {
+  const std::string p = "class_small_max_";
+  if (n.compare(0, p.size(), p) != 0) return -1;
+  const int k = std::atoi(n.c_str() + p.size());
+  return (k >= 0 && k < SIZE_CLASSES) ? k : -1;
+}
 }  // namespace
 
 int svo_ctx_set_option(svo_ctx* ctx, const char* name, int64_t value) {
@@ -149,6 +166,7 @@ int svo_ctx_set_option(svo_ctx* ctx, const char* name, int64_t value) {
     else if (n == "bitmap_bytes_per_cell") t.bitmap_bytes_per_cell = (uint32_t)value;
     else if (n == "batch_lanes") t.batch_lanes = (uint32_t)value;
     else if (n == "batch_min_meshes") t.batch_min_meshes = (uint32_t)value;
+    else if (class_option(n) >= 0) t.class_small_max[class_option(n)] = (uint32_t)value;
     else t.force_path = (int)value;
     return SVO_OK;
   }
@@ -169,6 +187,7 @@ int svo_ctx_get_option(svo_ctx* ctx, const char* name, int64_t* value) {
   else if (n == "dedup_path") *value = t.force_path;
   else if (n == "batch_lanes") *value = t.batch_lanes;
   else if (n == "batch_min_meshes") *value = t.batch_min_meshes;
+  else if (class_option(n) >= 0) *value = t.class_small_max[class_option(n)];
   else return SVO_E_INVALID;
   return SVO_OK;
 }
@@ -429,12 +448,86 @@ int svo_ctx_autotune(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params
     std::lock_guard<std::mutex> lk(C.mu);
     C.tuning = rc == SVO_OK ? best : base;
   }
+  // per size class: the class's meshes alone, timed on their voxelization
+  std::vector<std::vector<uint32_t>> members(SIZE_CLASSES);
+  for (uint32_t i = 0; i < n; ++i)
+    members[size_class_of(params[i].res, meshes[i].n_tris)].push_back(i);
+  const bool timing_was = C.timing;
+  C.timing = true;
+  auto vox_trial = [&](const std::vector<uint32_t>& sel, const Tuning& t, double& ms) -> int {
+    {
+      std::lock_guard<std::mutex> lk(C.mu);
+      C.tuning = t;
+    }
+    std::vector<svo_mesh_desc> d;
+    std::vector<svo_params> p;
+    for (uint32_t i : sel) {
+      d.push_back(meshes[i]);
+      p.push_back(params[i]);
+    }
+    std::vector<svo_tree*> tr(sel.size(), nullptr);
+    std::vector<int> s2(sel.size(), SVO_OK);
+    std::vector<double> times;
+    for (uint32_t r = 0; r <= reps; ++r) {
+      const int rc2 = svo_build_batch(ctx, d.data(), p.data(), (uint32_t)sel.size(),
+                                      meshes_on_device, tr.data(), s2.data());
+      for (auto& x : tr) {
+        svo_tree_free(x);
+        x = nullptr;
+      }
+      if (rc2 != SVO_OK) return rc2;
+      double v = 0;
+      {
+        std::lock_guard<std::mutex> lk(C.stage_mu);
+        for (size_t k = 0; k < C.stage_names.size(); ++k)
+          if (C.stage_names[k] == "classify" || C.stage_names[k] == "voxelize")
+            v += C.stage_ms[k];
+      }
+      if (r > 0) times.push_back(v);
+    }
+    std::sort(times.begin(), times.end());
+    ms = times[times.size() / 2];
+    return SVO_OK;
+  };
+  const uint32_t cands[] = {64, 128, 256, 512, 1024, 2048, 4096};
+  for (int c = 0; c < SIZE_CLASSES && rc == SVO_OK; ++c) {
+    if (members[c].empty()) continue;
+    Tuning t = best;
+    t.class_small_max[c] = 0;
+    double ref_ms = 0;
+    rc = vox_trial(members[c], t, ref_ms);
+    ++trials;
+    double cls_best = ref_ms;
+    uint32_t pick = 0;
+    for (uint32_t v : cands) {
+      if (rc != SVO_OK) break;
+      if (v == best.small_max_cells) continue;
+      t.class_small_max[c] = v;
+      double ms = 0;
+      rc = vox_trial(members[c], t, ms);
+      ++trials;
+      if (rc == SVO_OK && ms < cls_best) {
+        cls_best = ms;
+        pick = v;
+      }
+    }
+    if (rc == SVO_OK && pick && cls_best < 0.97 * ref_ms) best.class_small_max[c] = pick;
+  }
+  C.timing = timing_was;
+  {
+    std::lock_guard<std::mutex> lk(C.mu);
+    C.tuning = rc == SVO_OK ? best : base;
+  }
   if (out) {
     out->small_max_cells = best.small_max_cells;
     out->batch_lanes = best.batch_lanes;
     out->best_ms = best_ms;
     out->default_ms = default_ms;
     out->n_trials = trials;
+    for (int c = 0; c < SIZE_CLASSES; ++c) {
+      out->class_small_max[c] = best.class_small_max[c];
+      out->class_meshes[c] = (uint32_t)members[c].size();
+    }
   }
   return rc;
 }

@@ -439,7 +439,8 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
       const uint64_t nx = g1[0] - g0[0] + 1, ny = g1[1] - g0[1] + 1, nz = g1[2] - g0[2] + 1;
       const uint64_t c = nx * ny * nz;
       cells += c;
-      if (c > small_max) chunks += (ny * nz + BIG_ROWS - 1) / BIG_ROWS;
+      if (c > (md.small_max ? md.small_max : small_max))
+        chunks += (ny * nz + BIG_ROWS - 1) / BIG_ROWS;
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -505,7 +506,8 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
           est *= __fdividef(hi - lo, P.unit[a]) + 1.0f;
         }
         const uint32_t ce = !(est >= 1.0f) ? 1u : (est > 1e9f ? 1000000000u : (uint32_t)est);
-        cls = ce > small_max ? 15u : min(14u, 32u - (uint32_t)__builtin_clz(ce));  // 1 .. 14
+        const uint32_t sm = md.small_max ? md.small_max : small_max;
+        cls = ce > sm ? 15u : min(14u, 32u - (uint32_t)__builtin_clz(ce));  // 1 .. 14
       }
     }
     if (threadIdx.x < 16) s_hist[threadIdx.x] = 0;
@@ -532,7 +534,7 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
       const uint64_t nx = S.g1[0] - S.g0[0] + 1, ny = S.g1[1] - S.g0[1] + 1,
                      nz = S.g1[2] - S.g0[2] + 1;
       const uint64_t cells = nx * ny * nz;
-      if (cells <= small_max) {
+      if (cells <= (md.small_max ? md.small_max : small_max)) {
         count = cells;
       } else {
         const uint64_t nch = (ny * nz + BIG_ROWS - 1) / BIG_ROWS;

@@ -836,12 +836,18 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
       d.tri = dt;
     }
     for (int a = 0; a < 3; ++a) d.res[a] = mi.params.res[a];
+    d.small_max = plan.tuning.class_small_max[size_class_of(d.res, d.n_tris)];
     d.tri_base = tri_base[i];
     d.pt_base = 0;
     tri_base[i + 1] = tri_base[i] + d.n_tris;
     max_points = std::max<uint64_t>(max_points, d.n_points);
   }
   const uint64_t T = tri_base[n];
+  // per-class limits shrink with the pass's triangle count as the global one
+  // does (below)
+  for (MeshDev& d : md)
+    if (d.small_max)
+      d.small_max = (uint32_t)std::min<uint64_t>(d.small_max, std::max<uint64_t>(32, T / 128));
 
   // occupancy-bitmap words per mesh: one bit per Morton code below
   // morton(res - 1) + 1 (the reference's dense array size, SVO.cu:342-345)

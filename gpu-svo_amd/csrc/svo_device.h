@@ -159,7 +159,7 @@ struct MeshDev {
   uint64_t tri_base;  // first global triangle index of this mesh
   uint64_t pt_base;   // first global point index of this mesh
   int32_t res[3];
-  uint32_t pad;
+  uint32_t small_max;  // box cells voxelized by one lane for this mesh (0: the pass's)
 };
 
 // Big-triangle work item: a range of (y, z) rows of one triangle's box.

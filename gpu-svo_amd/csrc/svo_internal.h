@@ -10,6 +10,8 @@
 namespace svo {
 
 // ------------------------------------------------------------ tuning knobs
+constexpr int SIZE_CLASSES = 15;
+
 struct Tuning {
   uint32_t small_max_cells = 1024;  // bbox cells handled by one lane
   uint32_t chunk_cells = 4096;      // cells per big-triangle work item
@@ -25,7 +27,23 @@ struct Tuning {
   // up to batch_lanes concurrent passes
   uint32_t batch_lanes = 2;
   uint32_t batch_min_meshes = 16;
+  // Per size class (size_class_of) box-cell limit of the one-lane
+  // voxelization; 0 = small_max_cells.  Set by the tuning search.
+  uint32_t class_small_max[SIZE_CLASSES] = {};
 };
+
+// Size class of a mesh for tuning: 5 resolution bands (max axis <= 64, 128,
+// 256, 512, above) x 3 bands of res^2 / triangles, which grows with the
+// triangles' edge length in voxels for a surface spanning its box (<= 4:
+// triangles of about a voxel or less, <= 64: a few voxels, above: larger).
+inline int size_class_of(const int32_t res[3], uint64_t n_tris) {
+  const int32_t r = res[0] > res[1] ? (res[0] > res[2] ? res[0] : res[2])
+                                    : (res[1] > res[2] ? res[1] : res[2]);
+  const int band = r <= 64 ? 0 : r <= 128 ? 1 : r <= 256 ? 2 : r <= 512 ? 3 : 4;
+  const double q = (double)r * (double)r / (double)(n_tris ? n_tris : 1);
+  const int qb = q <= 4.0 ? 0 : q <= 64.0 ? 1 : 2;
+  return 3 * band + qb;
+}
 
 // ------------------------------------------------------------ k_voxelize.hip
 // Per-mesh ordered-float min/max of all points (6 x uint32 per mesh) and the

@@ -81,7 +81,8 @@ class Completion(C.Structure):
 
 class TuneResult(C.Structure):
     _fields_ = [("small_max_cells", C.c_uint32), ("batch_lanes", C.c_uint32),
-                ("best_ms", C.c_double), ("default_ms", C.c_double), ("n_trials", C.c_uint32)]
+                ("best_ms", C.c_double), ("default_ms", C.c_double), ("n_trials", C.c_uint32),
+                ("class_small_max", C.c_uint32 * 15), ("class_meshes", C.c_uint32 * 15)]
 
 
 # Every exported entry point and its ctypes signature (kept in step with
@@ -501,7 +502,10 @@ class Context:
         r = TuneResult()
         _check(lib().svo_ctx_autotune(self._h, descs, params, n, 1 if on_device else 0, reps,
                                       C.byref(r)), "autotune")
-        return {k: getattr(r, k) for k, _ in TuneResult._fields_}
+        out = {k: getattr(r, k) for k, _ in TuneResult._fields_[:5]}
+        out["classes"] = {k: {"meshes": r.class_meshes[k], "small_max_cells": r.class_small_max[k]}
+                          for k in range(15) if r.class_meshes[k]}
+        return out
 
     def save_tuning(self, path: str):
         _check(lib().svo_ctx_save_tuning(self._h, path.encode()), "save tuning")

@@ -187,15 +187,24 @@ int svo_plan_destroy(svo_plan* plan);
  * Unknown names or out-of-range values give SVO_E_INVALID. */
 int svo_ctx_set_option(svo_ctx* ctx, const char* name, int64_t value);
 int svo_ctx_get_option(svo_ctx* ctx, const char* name, int64_t* value);
-/* Tuning search (no counterpart in the reference): builds the sample batch
- * `reps` times (after one warm-up; the median time counts) under the current
- * options and under each combination of small_max_cells in {128, 256, 512, 1024, 2048} and
- * batch_lanes in {1, 2, 3, 4}, keeps the fastest on the context and reports
- * it.  The trees built are freed. */
+/* Tuning search (no counterpart in the reference), on a sample batch:
+ *  1. builds it `reps` times (after one warm-up; the median counts) under the
+ *     current options and under each combination of small_max_cells in
+ *     {128, 256, 512, 1024, 2048} and batch_lanes in {1, 2, 3, 4}, and keeps
+ *     the fastest;
+ *  2. then, per size class present in the sample (5 resolution bands x 3
+ *     bands of res^2 / triangles; the option names class_small_max_<k>),
+ *     builds that class's meshes alone with the one-lane box limit of the
+ *     class in {64, 128, 256, 512, 1024, 2048, 4096} and keeps the limit
+ *     whose voxelization (classify + voxelize stages, HIP events) is
+ *     fastest, when it beats the global limit by more than 3 %.
+ * The choices stay on the context; the trees built are freed. */
 typedef struct {
   uint32_t small_max_cells, batch_lanes; /* chosen */
   double best_ms, default_ms;            /* median wall ms per build of the sample */
   uint32_t n_trials;
+  uint32_t class_small_max[15];          /* per size class; 0 = small_max_cells */
+  uint32_t class_meshes[15];             /* sample meshes of each class */
 } svo_tune_result;
 int svo_ctx_autotune(svo_ctx* ctx, const svo_mesh_desc* meshes, const svo_params* params,
                      uint32_t n, int meshes_on_device, uint32_t reps, svo_tune_result* out);
