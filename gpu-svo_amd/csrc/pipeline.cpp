@@ -42,6 +42,18 @@ namespace svo {
 static inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // ------------------------------------------------------------------ context
+// Live contexts of the process: each caches freed blocks up to 80% of its
+// device, so an allocation that fails in one context first makes the others
+// on that device give their cached blocks back.
+static std::mutex g_contexts_mu;
+static std::vector<Context*> g_contexts;
+
+static void trim_peer_caches(const Context* self) {
+  std::lock_guard<std::mutex> lk(g_contexts_mu);
+  for (Context* c : g_contexts)
+    if (c != self && c->device == self->device) c->trim_cache(0);
+}
+
 int Context::init(int dev, uint64_t bud) {
   device = dev;
   budget = bud;
@@ -53,6 +65,8 @@ int Context::init(int dev, uint64_t bud) {
   // block maps its pages and costs seconds at 100+ GB)
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && total_b) cache_limit = total_b / 10 * 8;
+  std::lock_guard<std::mutex> lk(g_contexts_mu);
+  g_contexts.push_back(this);
   return SVO_OK;
 }
 
@@ -76,6 +90,10 @@ void Context::release_lanes() {
 }
 
 Context::~Context() {
+  {
+    std::lock_guard<std::mutex> lk(g_contexts_mu);
+    g_contexts.erase(std::remove(g_contexts.begin(), g_contexts.end(), this), g_contexts.end());
+  }
   release_lanes();
   hipSetDevice(device);
   if (aux_stream) hipStreamDestroy(aux_stream);
@@ -152,8 +170,15 @@ int Context::dev_malloc(void** p, size_t bytes) {
   hipSetDevice(device);
   if (hipMalloc(p, cls) != hipSuccess) {
     hipGetLastError();
-    trim_cache(0);  // give cached blocks back and retry once
-    if (hipMalloc(p, cls) != hipSuccess) {
+    trim_cache(0);  // give cached blocks back and retry
+    bool ok = hipMalloc(p, cls) == hipSuccess;
+    if (!ok) {
+      hipGetLastError();
+      trim_peer_caches(this);  // then those of the other contexts on the device
+      hipSetDevice(device);
+      ok = hipMalloc(p, cls) == hipSuccess;
+    }
+    if (!ok) {
       hipGetLastError();
       release(cls);
       *p = nullptr;

@@ -140,3 +140,27 @@ def test_plan_rejects_mixed_semantics(gpu_ctx):
     with pytest.raises(svo_amd.SvoError) as e:
         gpu_ctx.plan(meshes, semantics=[0, 1, 0, 1][:len(meshes)])
     assert e.value.status == svo_amd.E_INVALID
+
+
+def test_plan_with_64bit_keys_and_no_records(gpu_ctx):
+    """64-bit keys (composite keys of a batch past 32 bits) and builds without
+    vertex / edge records replay bit-exact as well."""
+    import svo_amd
+    meshes = _meshes(20, seed=54, res_choices=(128, 256))
+    for kb, flags in ((64, 0), (0, svo_amd.BUILD_NO_VERTEX_EDGE)):
+        print(f"plan key_bits {kb} flags {flags}", flush=True)
+        eager, st = gpu_ctx.build_batch(meshes, key_bits=kb, flags=flags)
+        assert all(s == 0 for s in st), st
+        want = _fields(eager)
+        for t in eager:
+            t.free()
+        plan = gpu_ctx.plan(meshes, key_bits=kb, flags=flags)
+        try:
+            for rep in range(2):
+                trees, st, replayed = plan.run()
+                assert replayed and all(s == 0 for s in st), st
+                _same(f"kb {kb} flags {flags} replay {rep}", _fields(trees), want)
+                for t in trees:
+                    t.free()
+        finally:
+            plan.close()
