@@ -28,7 +28,6 @@ constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / 64;
 constexpr int STAGE = 1024;        // keys staged per wave
 constexpr int BBOX_CHUNK = 4096;   // points per bbox block
-constexpr uint32_t CLASSIFY_BLOCKS = 1024;  // grid bound of k_classify
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -418,45 +417,113 @@ struct VoxOut {
 // and 128 rows measured slower: more items, each paying the triangle set-up).
 constexpr uint64_t BIG_ROWS = 512;
 
-__global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
-                                                    const MeshParams* params,
-                                                    const uint64_t* tri_base, uint32_t n_meshes,
-                                                    uint64_t n_tris, uint32_t small_max,
-                                                    uint32_t chunk_cells,
-                                                    unsigned long long* totals) {
-  // grid-stride over the triangles with a bounded grid: the two totals take
-  // one atomic per workgroup, and atomics on one address serialize
+// Voxelization order.  One lane walks each small triangle's box, so a wave
+// runs as long as the largest box among its 64 triangles.  k_classify sorts
+// each run of ORDER_GROUP consecutive triangles by box cells, largest first
+// (triangles past the one-lane limit, which k_vox_small only queues, then
+// eighth-octave classes, exact below 32 cells) into `order`: the waves of a
+// run see boxes of about one size.  Triangles of one class keep roughly their
+// input order (ranks are taken item by item), which keeps the neighbouring
+// triangles of a mesh together.  k_vox_small deals the runs' workgroup slots
+// out slot-major (the first slot of every run, then the second, ...), so the
+// workgroups of the largest boxes start first and the launch ends on the
+// smallest.  The order does not change the hits.
+constexpr uint32_t ORDER_THREADS = 1024;
+constexpr uint32_t ORDER_PER = 8;
+constexpr uint32_t ORDER_GROUP = ORDER_THREADS * ORDER_PER;
+constexpr uint32_t ORDER_BINS = 256;
+static_assert(ORDER_GROUP % BLOCK == 0, "a run fills whole workgroups of k_vox_small");
+
+__device__ inline uint32_t size_bin(uint64_t cells, uint64_t small_max) {
+  if (cells > small_max) return ORDER_BINS - 1;
+  if (cells < 32) return (uint32_t)cells;
+  const uint32_t e = 63u - (uint32_t)__builtin_clzll(cells);  // >= 5
+  const uint32_t b = 32u + 8u * (e - 5u) + (uint32_t)((cells >> (e - 3u)) & 7u);
+  return b < ORDER_BINS - 2 ? b : ORDER_BINS - 2;
+}
+
+// Candidate cells and big-triangle chunk counts (totals[0], totals[1]) and
+// the voxelization order (one workgroup per run of ORDER_GROUP triangles).
+__global__ __launch_bounds__(ORDER_THREADS) void k_classify(const MeshDev* meshes,
+                                                            const MeshParams* params,
+                                                            const uint64_t* tri_base,
+                                                            uint32_t n_meshes, uint64_t n_tris,
+                                                            uint32_t small_max,
+                                                            unsigned long long* totals,
+                                                            uint32_t* order) {
+  __shared__ uint32_t s_bin[ORDER_BINS];
+  __shared__ uint64_t red[ORDER_THREADS / 64][2];
+  for (uint32_t i = threadIdx.x; i < ORDER_BINS; i += ORDER_THREADS) s_bin[i] = 0;
+  const uint64_t g0 = (uint64_t)blockIdx.x * ORDER_GROUP;
+  uint32_t bin[ORDER_PER];
   uint64_t cells = 0, chunks = 0;
-  for (uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; gid < n_tris;
-       gid += (uint64_t)gridDim.x * BLOCK) {
-    const uint32_t m = find_mesh(tri_base, n_meshes, gid);
-    const MeshDev& md = meshes[m];
-    const MeshParams& P = params[m];
-    float v[3][3];
-    if (P.status == 0 && load_tri(md, P, gid - md.tri_base, v)) {
-      int g0[3], g1[3];
-      tri_box(v, P, g0, g1);
-      const uint64_t nx = g1[0] - g0[0] + 1, ny = g1[1] - g0[1] + 1, nz = g1[2] - g0[2] + 1;
-      const uint64_t c = nx * ny * nz;
-      cells += c;
-      if (c > (md.small_max ? md.small_max : small_max))
-        chunks += (ny * nz + BIG_ROWS - 1) / BIG_ROWS;
+#pragma unroll
+  for (uint32_t k = 0; k < ORDER_PER; ++k) {
+    const uint64_t gid = g0 + k * ORDER_THREADS + threadIdx.x;
+    bin[k] = ORDER_BINS - 1;  // a failed mesh or a bad index: nothing to walk, last
+    if (gid < n_tris) {
+      const uint32_t m = find_mesh(tri_base, n_meshes, gid);
+      const MeshDev& md = meshes[m];
+      const MeshParams& P = params[m];
+      float v[3][3];
+      if (P.status == 0 && load_tri(md, P, gid - md.tri_base, v)) {
+        int g0b[3], g1b[3];
+        tri_box(v, P, g0b, g1b);
+        const uint64_t nx = g1b[0] - g0b[0] + 1, ny = g1b[1] - g0b[1] + 1,
+                       nz = g1b[2] - g0b[2] + 1;
+        const uint64_t c = nx * ny * nz;
+        const uint64_t sm = md.small_max ? md.small_max : small_max;
+        cells += c;
+        if (c > sm) chunks += (ny * nz + BIG_ROWS - 1) / BIG_ROWS;
+        bin[k] = ORDER_BINS - 1 - size_bin(c, sm);  // largest first
+      }
     }
   }
+  __syncthreads();  // s_bin cleared
+  uint32_t rank[ORDER_PER];
+#pragma unroll
+  for (uint32_t k = 0; k < ORDER_PER; ++k)
+    rank[k] = g0 + k * ORDER_THREADS + threadIdx.x < n_tris ? atomicAdd(&s_bin[bin[k]], 1u) : 0u;
   for (int off = 32; off > 0; off >>= 1) {
     cells += __shfl_xor(cells, off);
     chunks += __shfl_xor(chunks, off);
   }
-  __shared__ uint64_t red[WAVES][2];
   if ((threadIdx.x & 63) == 0) {
     red[threadIdx.x >> 6][0] = cells;
     red[threadIdx.x >> 6][1] = chunks;
   }
+  __syncthreads();  // bin counts and wave sums complete
+  if (threadIdx.x < 64) {
+    // exclusive scan of the bin counts, four bins per lane
+    const uint32_t l = threadIdx.x;
+    uint32_t c[4], t = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      c[j] = s_bin[4 * l + j];
+      t += c[j];
+    }
+    uint32_t inc = t;
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t o = __shfl_up(inc, d);
+      if (l >= (uint32_t)d) inc += o;
+    }
+    uint32_t ex = inc - t;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      s_bin[4 * l + j] = ex;
+      ex += c[j];
+    }
+    if (l < 2) {
+      uint64_t s = 0;
+      for (uint32_t w = 0; w < ORDER_THREADS / 64; ++w) s += red[w][l];
+      if (s) atomicAdd(totals + l, (unsigned long long)s);
+    }
+  }
   __syncthreads();
-  if (threadIdx.x < 2) {
-    uint64_t s = 0;
-    for (int w = 0; w < WAVES; ++w) s += red[w][threadIdx.x];
-    if (s) atomicAdd(totals + threadIdx.x, (unsigned long long)s);
+#pragma unroll
+  for (uint32_t k = 0; k < ORDER_PER; ++k) {
+    const uint64_t gid = g0 + k * ORDER_THREADS + threadIdx.x;
+    if (gid < n_tris) order[g0 + s_bin[bin[k]] + rank[k]] = (uint32_t)gid;
   }
 }
 
@@ -474,7 +541,7 @@ __global__ void k_flag_bad_index(const MeshDev* meshes, MeshParams* params,
 template <typename K, bool BITMAP>
 __global__ __launch_bounds__(BLOCK) void k_vox_small(
     const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base, uint32_t n_meshes,
-    uint64_t n_tris, int cb, uint32_t small_max, uint32_t chunk_cells, VoxOut<K> out,
+    uint64_t n_tris, int cb, uint32_t small_max, const uint32_t* order, VoxOut<K> out,
     BigItem* big, unsigned long long* big_count, uint64_t big_cap) {
   __shared__ K stage[BITMAP ? 1 : WAVES][BITMAP ? 1 : STAGE];
   Stager<K> st;
@@ -484,42 +551,12 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
   BitSink bs;
   bs.bits = out.bits;
   bs.count = out.count;
-  // The workgroup's 256 triangles are handed to its lanes in order of box
-  // size (log2 classes of an estimate of the box cells; estimated big ones
-  // last): a wave's lanes then walk boxes of about one size instead of
-  // waiting for the largest of 64 random ones.  The order of the triangles
-  // does not change the hits.
-  __shared__ uint32_t s_hist[16], s_perm[BLOCK];
-  {
-    const uint64_t own = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
-    uint32_t cls = 0;
-    if (own < n_tris) {
-      const uint32_t m = find_mesh(tri_base, n_meshes, own);
-      const MeshDev& md = meshes[m];
-      const MeshParams& P = params[m];
-      float v[3][3];
-      if (P.status == 0 && load_tri(md, P, own - md.tri_base, v)) {
-        float est = 1.0f;
-        for (int a = 0; a < 3; ++a) {
-          const float lo = fminf(v[0][a], fminf(v[1][a], v[2][a]));
-          const float hi = fmaxf(v[0][a], fmaxf(v[1][a], v[2][a]));
-          est *= __fdividef(hi - lo, P.unit[a]) + 1.0f;
-        }
-        const uint32_t ce = !(est >= 1.0f) ? 1u : (est > 1e9f ? 1000000000u : (uint32_t)est);
-        const uint32_t sm = md.small_max ? md.small_max : small_max;
-        cls = ce > sm ? 15u : min(14u, 32u - (uint32_t)__builtin_clz(ce));  // 1 .. 14
-      }
-    }
-    if (threadIdx.x < 16) s_hist[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t rank = atomicAdd(&s_hist[cls], 1u);
-    __syncthreads();
-    uint32_t at = rank;
-    for (uint32_t c = 0; c < cls; ++c) at += s_hist[c];
-    s_perm[at] = threadIdx.x;
-    __syncthreads();
-  }
-  const uint64_t gid = (uint64_t)blockIdx.x * BLOCK + s_perm[threadIdx.x];
+  // triangles in the order k_classify wrote (boxes of about one size per
+  // wave), workgroup slots of the runs dealt out slot-major
+  const uint32_t runs = (uint32_t)((n_tris + ORDER_GROUP - 1) / ORDER_GROUP);
+  const uint64_t pos = (uint64_t)(blockIdx.x % runs) * ORDER_GROUP +
+                       (uint64_t)(blockIdx.x / runs) * BLOCK + threadIdx.x;
+  const uint64_t gid = pos < n_tris ? order[pos] : n_tris;
   TriSetup S;
   uint64_t count = 0;
   K mesh_key = 0;
@@ -673,29 +710,31 @@ void launch_params(const MeshDev* meshes, uint32_t n_meshes, const uint32_t* acc
 
 void launch_classify(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
                      uint32_t n_meshes, uint64_t n_tris, const Tuning& tn,
-                     unsigned long long* totals, hipStream_t s) {
+                     unsigned long long* totals, uint32_t* order, hipStream_t s) {
   if (n_tris == 0) return;
   const uint32_t blocks = (uint32_t)((n_tris + BLOCK - 1) / BLOCK);
   hipLaunchKernelGGL(k_flag_bad_index, dim3(blocks), dim3(BLOCK), 0, s, meshes,
                      const_cast<MeshParams*>(params), tri_base, n_meshes, n_tris);
-  hipLaunchKernelGGL(k_classify, dim3(std::min<uint32_t>(blocks, CLASSIFY_BLOCKS)), dim3(BLOCK), 0,
-                     s, meshes, params, tri_base, n_meshes, n_tris, tn.small_max_cells,
-                     tn.chunk_cells, totals);
+  const uint32_t groups = (uint32_t)((n_tris + ORDER_GROUP - 1) / ORDER_GROUP);
+  hipLaunchKernelGGL(k_classify, dim3(groups), dim3(ORDER_THREADS), 0, s, meshes, params,
+                     tri_base, n_meshes, n_tris, tn.small_max_cells, totals, order);
 }
 
 template <typename K>
 void launch_voxelize(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
                      uint32_t n_meshes, uint64_t n_tris, int cb, const Tuning& tn, K* keys,
                      uint32_t* bitmap, const uint64_t* bm_off, unsigned long long* key_count,
-                     BigItem* big, unsigned long long* big_count, uint64_t big_cap,
-                     hipStream_t s) {
+                     const uint32_t* order, BigItem* big, unsigned long long* big_count,
+                     uint64_t big_cap, hipStream_t s) {
   if (n_tris == 0) return;
   const VoxOut<K> out{keys, bitmap, bm_off, key_count};
-  const uint32_t blocks = (uint32_t)((n_tris + BLOCK - 1) / BLOCK);
+  // every run of ORDER_GROUP triangles has ORDER_GROUP / BLOCK workgroup slots
+  const uint32_t blocks = (uint32_t)((n_tris + ORDER_GROUP - 1) / ORDER_GROUP) *
+                          (ORDER_GROUP / BLOCK);
   const uint32_t bblocks = (uint32_t)((big_cap + WAVES - 1) / WAVES);
   if (bitmap) {
     hipLaunchKernelGGL((k_vox_small<K, true>), dim3(blocks), dim3(BLOCK), 0, s, meshes, params,
-                       tri_base, n_meshes, n_tris, cb, tn.small_max_cells, tn.chunk_cells, out,
+                       tri_base, n_meshes, n_tris, cb, tn.small_max_cells, order, out,
                        big, big_count, big_cap);
     if (big_cap)
       hipLaunchKernelGGL((k_vox_big<K, true>), dim3(bblocks), dim3(BLOCK), 0, s, meshes, params,
@@ -703,7 +742,7 @@ void launch_voxelize(const MeshDev* meshes, const MeshParams* params, const uint
                          (const unsigned long long*)big_count, n_tris, out);
   } else {
     hipLaunchKernelGGL((k_vox_small<K, false>), dim3(blocks), dim3(BLOCK), 0, s, meshes, params,
-                       tri_base, n_meshes, n_tris, cb, tn.small_max_cells, tn.chunk_cells, out,
+                       tri_base, n_meshes, n_tris, cb, tn.small_max_cells, order, out,
                        big, big_count, big_cap);
     if (big_cap)
       hipLaunchKernelGGL((k_vox_big<K, false>), dim3(bblocks), dim3(BLOCK), 0, s, meshes, params,
@@ -715,10 +754,12 @@ void launch_voxelize(const MeshDev* meshes, const MeshParams* params, const uint
 template void launch_voxelize<uint32_t>(const MeshDev*, const MeshParams*, const uint64_t*,
                                         uint32_t, uint64_t, int, const Tuning&, uint32_t*,
                                         uint32_t*, const uint64_t*, unsigned long long*,
-                                        BigItem*, unsigned long long*, uint64_t, hipStream_t);
+                                        const uint32_t*, BigItem*, unsigned long long*, uint64_t,
+                                        hipStream_t);
 template void launch_voxelize<uint64_t>(const MeshDev*, const MeshParams*, const uint64_t*,
                                         uint32_t, uint64_t, int, const Tuning&, uint64_t*,
                                         uint32_t*, const uint64_t*, unsigned long long*,
-                                        BigItem*, unsigned long long*, uint64_t, hipStream_t);
+                                        const uint32_t*, BigItem*, unsigned long long*, uint64_t,
+                                        hipStream_t);
 
 }  // namespace svo

@@ -172,7 +172,8 @@ int svo_k_voxelize(int device, const float* xyz, uint64_t n_points, const int32_
   uint32_t* acc = (uint32_t*)B.get(32);
   MeshParams* par = (MeshParams*)B.get(sizeof(MeshParams));
   unsigned long long* cnt = (unsigned long long*)B.get(64);
-  if (!dp || !dt || !md || !tb || !acc || !par || !cnt) return SVO_E_OOM;
+  uint32_t* order = (uint32_t*)B.get(4 * n_tris);
+  if (!dp || !dt || !md || !tb || !acc || !par || !cnt || !order) return SVO_E_OOM;
   MeshDev h{};
   h.pts = dp;
   h.tri = dt;
@@ -193,7 +194,7 @@ int svo_k_voxelize(int device, const float* xyz, uint64_t n_points, const int32_
       hipMemcpyAsync(cnt, zero, 64, hipMemcpyHostToDevice, s) == hipSuccess) {
     launch_bbox(md, 1, n_points, acc, s);
     launch_params(md, 1, acc, par, s);
-    launch_classify(md, par, tb, 1, n_tris, tn, cnt, s);
+    launch_classify(md, par, tb, 1, n_tris, tn, cnt, order, s);
     if (hipMemcpyAsync(tot, cnt, 32, hipMemcpyDeviceToHost, s) == hipSuccess &&
         hipStreamSynchronize(s) == hipSuccess)
       st = SVO_OK;
@@ -215,10 +216,10 @@ int svo_k_voxelize(int device, const float* xyz, uint64_t n_points, const int32_
     } else {
       if (kb == 64)
         launch_voxelize<uint64_t>(md, par, tb, 1, n_tris, 3 * bits, tn, (uint64_t*)keys, nullptr,
-                                  nullptr, cnt + 2, big, cnt + 3, n_big, s);
+                                  nullptr, cnt + 2, order, big, cnt + 3, n_big, s);
       else
         launch_voxelize<uint32_t>(md, par, tb, 1, n_tris, 3 * bits, tn, (uint32_t*)keys, nullptr,
-                                  nullptr, cnt + 2, big, cnt + 3, n_big, s);
+                                  nullptr, cnt + 2, order, big, cnt + 3, n_big, s);
       st = SVO_E_HIP;
       if (hipMemcpyAsync(tot, cnt, 32, hipMemcpyDeviceToHost, s) == hipSuccess &&
           hipStreamSynchronize(s) == hipSuccess) {

@@ -327,7 +327,7 @@ uint64_t estimate_bytes(uint64_t n_points, uint64_t n_tris, const svo_params* p)
   const double per_node = (p->flags & SVO_BUILD_NO_VERTEX_EDGE) ? 170.0
                           : (p->semantics == SVO_SEM_INTENDED ? 800.0 : 640.0);
   const double b = nodes * per_node + hits * 8.0 * 2.5 + (double)n_points * 12.0 +
-                   (double)n_tris * 12.0 + (1 << 20);
+                   (double)n_tris * 16.0 + (1 << 20);
   return (uint64_t)b;
 }
 
@@ -366,7 +366,7 @@ double pass_bytes_bound(const svo_params* p, int kb, uint64_t P, uint64_t T, int
     lv += K * cap + (l >= 2 ? 8.0 * cap : 0.0) + 3 * 256.0;
   }
   const double n_big = (double)T + rows / 512.0;
-  const double ws_common = 12.0 * P + 12.0 * T + 2 * 256.0 + lv + (double)level_tmp_bytes(kc) +
+  const double ws_common = 12.0 * P + 16.0 * T + 3 * 256.0 + lv + (double)level_tmp_bytes(kc) +
                            sizeof(BigItem) * std::max(n_big, 1.0) + (double)attr_tmp_bytes(4096) +
                            (1 << 20);  // control buffers of one mesh
   const double ws_bitmap = 4.0 * bm_words + (double)bitmap_level_tmp_bytes(bm_words);
@@ -868,6 +868,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     max_points = std::max<uint64_t>(max_points, d.n_points);
   }
   const uint64_t T = tri_base[n];
+  if (T >= (1ull << 32)) return SVO_E_RES_LIMIT;  // triangle indices are 32-bit on the device
   // per-class limits shrink with the pass's triangle count as the global one
   // does (below)
   for (MeshDev& d : md)
@@ -927,7 +928,9 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   Tuning tn = plan.tuning;
   tn.small_max_cells = (uint32_t)std::min<uint64_t>(tn.small_max_cells,
                                                     std::max<uint64_t>(32, T / 128));
-  launch_classify(d_md, d_par, d_tb, n, T, tn, d_cnt, s);
+  uint32_t* order = nullptr;
+  SVO_TRY(W.get("order", 4 * std::max<uint64_t>(T, 1), (void**)&order));
+  launch_classify(d_md, d_par, d_tb, n, T, tn, d_cnt, order, s);
   timer.mark("classify");
   uint64_t cells = 0, n_big = 0;
   if (capture) {
@@ -1053,7 +1056,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     clean = 0;
   }
   launch_voxelize<K>(d_md, d_par, d_tb, n, T, plan.cb, tn, keys_a, bitmap, d_bo,
-                     d_cnt + 2, big, d_cnt + 3, n_big, s);
+                     d_cnt + 2, order, big, d_cnt + 3, n_big, s);
   timer.mark("voxelize");
   if (use_bm) {
     launch_level_first_bitmap<K>(bitmap, bm_words, d_bo, n, kl, lv[1], lv[2], d_counts, d_spans,

@@ -14,7 +14,6 @@ constexpr int SIZE_CLASSES = 15;
 
 struct Tuning {
   uint32_t small_max_cells = 1024;  // bbox cells handled by one lane
-  uint32_t chunk_cells = 4096;      // cells per big-triangle work item
   uint32_t sort_blocks = 65536;     // grid bound of the radix sort passes (tiles stride)
   uint32_t scan_blocks = 1024;      // fixed grid of the level / attribute scans
   // Occupancy bitmap instead of the sort (level 1 straight from the bitmap)
@@ -52,10 +51,13 @@ void launch_bbox(const MeshDev* meshes, uint32_t n_meshes, uint64_t max_points, 
                  hipStream_t s);
 void launch_params(const MeshDev* meshes, uint32_t n_meshes, const uint32_t* acc,
                    MeshParams* params, hipStream_t s);
-// Candidate cells and big-triangle chunk counts (2 x uint64 in `totals`).
+// Candidate cells and big-triangle chunk counts (2 x uint64 added to
+// `totals`), and the order launch_voxelize takes the triangles in
+// (`order`: n_tris uint32, a permutation of the triangle indices that
+// groups triangles of about one box size).
 void launch_classify(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
                      uint32_t n_meshes, uint64_t n_tris, const Tuning& tn,
-                     unsigned long long* totals, hipStream_t s);
+                     unsigned long long* totals, uint32_t* order, hipStream_t s);
 // Surface voxelization (SVO.cu:37-188): emits composite keys.
 template <typename K>
 // Hits go to `keys` (raw, for the sort) when `bitmap` is null, else into the
@@ -64,8 +66,8 @@ template <typename K>
 void launch_voxelize(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
                      uint32_t n_meshes, uint64_t n_tris, int cb, const Tuning& tn, K* keys,
                      uint32_t* bitmap, const uint64_t* bm_off, unsigned long long* key_count,
-                     BigItem* big, unsigned long long* big_count, uint64_t big_cap,
-                     hipStream_t s);
+                     const uint32_t* order, BigItem* big, unsigned long long* big_count,
+                     uint64_t big_cap, hipStream_t s);
 
 // ------------------------------------------------------------ k_sort.hip
 // LSD radix sort of n (device-resident count) keys on bits [0, bits).
