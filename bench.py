@@ -207,6 +207,7 @@ def run_batch_workload(args):
     meshes = meshgen.mixed_batch(256, seed=3)
     n_tris = sum(int(T.shape[0]) for _, T, _ in meshes)
     ctx = svo_amd.Context(0)
+    tuning = load_tuning(ctx, args)  # explicit options below take precedence
     if args.batch_lanes:
         ctx.set_option("batch_lanes", args.batch_lanes)
     for opt in args.option:
@@ -214,7 +215,6 @@ def run_batch_workload(args):
         ctx.set_option(name, int(value))
     dms = [ctx.upload_mesh(P, T) for P, T, _ in meshes]
     descs = [dm.desc(res) for dm, (_, _, res) in zip(dms, meshes)]
-    tuning = load_tuning(ctx, args)
     if args.autotune:
         log("bench: tuning search on the batch (untimed)")
         tuning = {"search": ctx.autotune(descs, on_device=True, reps=5)}
@@ -378,6 +378,7 @@ def run_single_workload(args, name, P, T, res, key_bits, flags=0, desc=""):
     """One large mesh per step (plain path), inputs resident in HBM."""
     import svo_amd
     ctx = svo_amd.Context(0)
+    tuning = load_tuning(ctx, args)  # explicit options below take precedence
     for opt in args.option:
         oname, value = opt.split("=", 1)
         ctx.set_option(oname, int(value))
@@ -417,6 +418,7 @@ def run_single_workload(args, name, P, T, res, key_bits, flags=0, desc=""):
         "dedup_gbs": dedup_gbs(stats, stages),
         "compulsory_gbs": compulsory_gbs(stats, ms),
         "options": {o.split("=", 1)[0]: int(o.split("=", 1)[1]) for o in args.option} or None,
+        "tuning": tuning,
         "dedup_path": "bitmap" if stats["bitmap_bytes"] else "sort",
         "roofline": _roofline(stats, stages),
     }
@@ -666,7 +668,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="batch workload: skip the batch-plan (HIP graph) leg")
     ap.add_argument("--tuning", default=TUNING_FILE,
-                    help="tuning file loaded on the batch context ('none': library defaults)")
+                    help="tuning file loaded on the build context before the --option "
+                         "values ('none': library defaults)")
     ap.add_argument("--autotune", action="store_true",
                     help="batch workload: run the library's tuning search on the batch first")
     ap.add_argument("--batch-lanes", type=int, default=0,

@@ -54,12 +54,81 @@ static void trim_peer_caches(const Context* self) {
     if (c != self && c->device == self->device) c->trim_cache(0);
 }
 
+// Lane streams.  A HIP stream is bound at creation to one of the device's
+// hardware queues (GPU_MAX_HW_QUEUES of them, 4 by default); streams made
+// beyond that share queues, and the work of streams on one queue runs one
+// after the other.  The concurrent passes of a batch overlap only when their
+// lanes' streams sit on different queues, which holds for the first streams
+// a process makes but not for streams made after others were destroyed (a
+// second context's four lanes were measured on two queues, pairwise
+// serialized).  So lane streams are made once per process and device, the
+// first request making a burst of LANE_BURST main streams and then their
+// side streams, before any other stream of the library (the first context
+// on the device makes the burst before its own copy stream), and contexts
+// borrow them: a context gives its lanes' streams back when it releases its
+// lanes, and the next context reuses them with their queues.
+namespace {
+constexpr int LANE_BURST = 4;
+struct PooledStreams {
+  hipStream_t stream, side;
+  bool busy;
+};
+std::mutex g_stream_pool_mu;
+std::map<int, std::vector<PooledStreams>> g_stream_pool;
+
+// Appends n (stream, side) pairs to the pool of the current device: the n
+// main streams first, then the side streams.  Caller holds g_stream_pool_mu.
+int grow_pool(std::vector<PooledStreams>& pool, int n) {
+  std::vector<hipStream_t> made;
+  for (int i = 0; i < 2 * n; ++i) {
+    hipStream_t q = nullptr;
+    if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) {
+      for (hipStream_t m : made) hipStreamDestroy(m);
+      return SVO_E_HIP;
+    }
+    made.push_back(q);
+  }
+  for (int i = 0; i < n; ++i) pool.push_back({made[i], made[n + i], false});
+  return SVO_OK;
+}
+
+// Makes the first burst of `dev` (the current device) if it has none yet.
+int prepare_streams(int dev) {
+  std::lock_guard<std::mutex> lk(g_stream_pool_mu);
+  std::vector<PooledStreams>& pool = g_stream_pool[dev];
+  return pool.empty() ? grow_pool(pool, LANE_BURST) : SVO_OK;
+}
+
+// Lends a (stream, side) pair of `dev` (the current device) to a lane.
+int take_streams(int dev, hipStream_t& stream, hipStream_t& side) {
+  std::lock_guard<std::mutex> lk(g_stream_pool_mu);
+  std::vector<PooledStreams>& pool = g_stream_pool[dev];
+  size_t i = 0;
+  while (i < pool.size() && pool[i].busy) ++i;
+  if (i == pool.size()) {
+    const int st = grow_pool(pool, pool.empty() ? LANE_BURST : 1);
+    if (st != SVO_OK) return st;
+  }
+  pool[i].busy = true;
+  stream = pool[i].stream;
+  side = pool[i].side;
+  return SVO_OK;
+}
+
+void give_streams(int dev, hipStream_t stream) {
+  std::lock_guard<std::mutex> lk(g_stream_pool_mu);
+  for (PooledStreams& p : g_stream_pool[dev])
+    if (p.stream == stream) p.busy = false;
+}
+}  // namespace
+
 int Context::init(int dev, uint64_t bud) {
   device = dev;
   budget = bud;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || dev < 0 || dev >= n) return SVO_E_HIP;
   HIP_TRY(hipSetDevice(dev));
+  SVO_TRY(prepare_streams(dev));  // the lanes' streams take the first queues
   HIP_TRY(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
   // freed blocks stay cached up to 80% of the device (hipMalloc of a large
   // block maps its pages and costs seconds at 100+ GB)
@@ -81,8 +150,7 @@ void Context::release_lanes() {
     if (l->fork) hipEventDestroy(l->fork);
     if (l->join) hipEventDestroy(l->join);
     if (l->pinned) hipHostFree(l->pinned);
-    if (l->stream) hipStreamDestroy(l->stream);
-    if (l->side) hipStreamDestroy(l->side);
+    if (l->stream) give_streams(device, l->stream);
   }
   lanes.clear();
   free_lanes.clear();
@@ -105,12 +173,10 @@ Lane* Context::acquire_lane() {
   if (free_lanes.empty() && lanes.size() < 8) {
     auto l = std::make_unique<Lane>();
     hipSetDevice(device);
-    if (hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&l->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&l->fork, hipEventDisableTiming) != hipSuccess ||
+    if (take_streams(device, l->stream, l->side) != SVO_OK) return nullptr;
+    if (hipEventCreateWithFlags(&l->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&l->join, hipEventDisableTiming) != hipSuccess) {
-      if (l->stream) hipStreamDestroy(l->stream);
-      if (l->side) hipStreamDestroy(l->side);
+      give_streams(device, l->stream);
       if (l->fork) hipEventDestroy(l->fork);
       if (l->join) hipEventDestroy(l->join);
       return nullptr;
