@@ -21,6 +21,7 @@ its p50 / p99 latency go into the same line.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -249,10 +250,18 @@ def run_batch_workload(args):
     # timed steps without stage events; the same number of steps with HIP
     # events on every stage follows, for the stage times and the roofline
     log(f"bench: timing {args.steps} batched steps")
+    step_ms = []
+    mallocs0 = ctx.device_mallocs()
+    gc.collect()
+    gc.disable()  # no collector pause inside the timed steps
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        ts = time.perf_counter()
         nv, nn = batch_step()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
     t1 = time.perf_counter()
+    gc.enable()
+    mallocs_timed = ctx.device_mallocs() - mallocs0
     ms_batch = (t1 - t0) * 1e3 / args.steps
     log(f"bench: {args.steps} batched steps with stage events")
     ctx.set_timing(True)
@@ -314,6 +323,8 @@ def run_batch_workload(args):
         "warmup": args.warmup,
         "ms_per_step": round(ms_batch, 3),
         "ms_per_step_with_stage_events": round(ms_events, 3),
+        "step_ms": [round(x, 3) for x in step_ms],
+        "device_mallocs_in_timed_steps": mallocs_timed,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -648,8 +659,8 @@ def copy_probe(nbytes):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--workload", default="batch",
                     choices=["batch", "single", "soup", "soup2048", "served", "io"])

@@ -211,6 +211,10 @@ uint64_t svo_ctx_workspace_bytes(const svo_ctx* ctx) {
   return ctx ? ctx->impl->ws_bytes.load() : 0;
 }
 
+uint64_t svo_ctx_device_mallocs(const svo_ctx* ctx) {
+  return ctx ? ctx->impl->n_mallocs.load() : 0;
+}
+
 int svo_ctx_last_pass_stats(svo_ctx* ctx, svo_pass_stats* out) {
   if (!ctx || !out) return SVO_E_INVALID;
   std::lock_guard<std::mutex> lk(ctx->impl->stage_mu);

@@ -168,24 +168,41 @@ Context::~Context() {
   aux_stream = nullptr;
 }
 
-Lane* Context::acquire_lane() {
+// A batch's sub-batch pass g asks for lane g: the sub-batches of a batch
+// are the same from call to call, so each lane's workspaces fit its pass
+// after the first call instead of growing (and allocating) whenever a pass
+// lands on a lane that last ran a smaller one.  A busy preferred lane (other
+// threads building on the context) yields any free one.
+Lane* Context::acquire_lane(int prefer) {
   std::unique_lock<std::mutex> lk(mu);
-  if (free_lanes.empty() && lanes.size() < 8) {
+  auto make_lane = [&]() -> bool {
     auto l = std::make_unique<Lane>();
     hipSetDevice(device);
-    if (take_streams(device, l->stream, l->side) != SVO_OK) return nullptr;
+    if (take_streams(device, l->stream, l->side) != SVO_OK) return false;
     if (hipEventCreateWithFlags(&l->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&l->join, hipEventDisableTiming) != hipSuccess) {
       give_streams(device, l->stream);
       if (l->fork) hipEventDestroy(l->fork);
       if (l->join) hipEventDestroy(l->join);
-      return nullptr;
+      return false;
     }
     l->ws.ctx = this;
     free_lanes.push_back(l.get());
     lanes.push_back(std::move(l));
-  }
+    return true;
+  };
+  while (prefer >= 0 && (size_t)prefer >= lanes.size() && lanes.size() < MAX_LANES)
+    if (!make_lane()) return nullptr;
+  if (free_lanes.empty() && lanes.size() < MAX_LANES && !make_lane()) return nullptr;
   cv.wait(lk, [&] { return !free_lanes.empty(); });
+  if (prefer >= 0 && (size_t)prefer < lanes.size()) {
+    auto it = std::find(free_lanes.begin(), free_lanes.end(), lanes[prefer].get());
+    if (it != free_lanes.end()) {
+      Lane* l = *it;
+      free_lanes.erase(it);
+      return l;
+    }
+  }
   Lane* l = free_lanes.back();
   free_lanes.pop_back();
   return l;
@@ -251,6 +268,7 @@ int Context::dev_malloc(void** p, size_t bytes) {
       return SVO_E_OOM;
     }
   }
+  n_mallocs.fetch_add(1);
   return SVO_OK;
 }
 
@@ -1500,7 +1518,8 @@ int run_plan_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const 
 
 // One pipeline pass over the valid meshes `sel` of `in` on one lane.
 int run_pass(Context& C, const std::vector<MeshInput>& in, const std::vector<uint32_t>& sel,
-             std::vector<svo_tree*>& trees, std::vector<int>& statuses, CallTiming& ct) {
+             std::vector<svo_tree*>& trees, std::vector<int>& statuses, CallTiming& ct,
+             int lane_pref = -1) {
   struct ActivePass {
     Context& c;
     explicit ActivePass(Context& x) : c(x) { c.active_passes.fetch_add(1); }
@@ -1513,7 +1532,7 @@ int run_pass(Context& C, const std::vector<MeshInput>& in, const std::vector<uin
     for (uint32_t i : plan.idx) statuses[i] = SVO_E_RES_LIMIT;
     return SVO_E_RES_LIMIT;
   }
-  Lane* L = C.acquire_lane();
+  Lane* L = C.acquire_lane(lane_pref);
   if (!L) return SVO_E_HIP;
   hipSetDevice(C.device);
   int st = run_plan_typed(C, *L, in, plan, trees, statuses);
@@ -1576,8 +1595,8 @@ int run_set(Context& C, const std::vector<MeshInput>& in, const std::vector<uint
   std::vector<int> rc(groups, SVO_OK);
   std::vector<std::thread> th;
   for (uint32_t g = 1; g < groups; ++g)
-    th.emplace_back([&, g] { rc[g] = run_pass(C, in, sel[g], trees, statuses, ct); });
-  rc[0] = run_pass(C, in, sel[0], trees, statuses, ct);
+    th.emplace_back([&, g] { rc[g] = run_pass(C, in, sel[g], trees, statuses, ct, (int)g); });
+  rc[0] = run_pass(C, in, sel[0], trees, statuses, ct, 0);
   for (auto& t : th) t.join();
   for (int r : rc)
     if (r != SVO_OK) return r;

@@ -100,7 +100,10 @@ struct Context : std::enable_shared_from_this<Context> {
   // their workspaces) and trims the block cache; idempotent.
   void release_lanes();
   ~Context();
-  Lane* acquire_lane();
+  static constexpr size_t MAX_LANES = 8;
+  // A free lane, lane `prefer` if it is free (made on demand, at most
+  // MAX_LANES); waits while all lanes are busy.
+  Lane* acquire_lane(int prefer = -1);
   void release_lane(Lane* l);
   int reserve(uint64_t bytes);  // budget accounting; SVO_E_OOM if over
   void release(uint64_t bytes);
@@ -113,6 +116,7 @@ struct Context : std::enable_shared_from_this<Context> {
   std::mutex cache_mu;
   std::multimap<size_t, void*> cache;
   uint64_t cached_bytes = 0;
+  std::atomic<uint64_t> n_mallocs{0};  // hipMalloc calls (cache misses)
   uint64_t cache_limit = 96ull << 30;
 };
 

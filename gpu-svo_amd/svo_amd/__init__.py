@@ -107,6 +107,7 @@ _SIGS = {
                                C.POINTER(C.c_int)]),
     "svo_plan_destroy": (C.c_int, [C.c_void_p]),
     "svo_ctx_workspace_bytes": (C.c_uint64, [C.c_void_p]),
+    "svo_ctx_device_mallocs": (C.c_uint64, [C.c_void_p]),
     "svo_device_alloc": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
     "svo_device_free": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64]),
     "svo_device_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]),
@@ -433,6 +434,10 @@ class Context:
     def workspace_bytes(self) -> int:
         """The part of bytes_in_use() held by the lanes' workspaces."""
         return int(lib().svo_ctx_workspace_bytes(self._h))
+
+    def device_mallocs(self) -> int:
+        """hipMalloc calls the context has made (its block cache's misses)."""
+        return int(lib().svo_ctx_device_mallocs(self._h))
 
     def upload_mesh(self, points, tris) -> "DeviceMesh":
         """Copies a mesh to device memory (kept until DeviceMesh.free)."""

@@ -227,6 +227,10 @@ uint64_t svo_ctx_bytes_in_use(const svo_ctx* ctx);
 /* The part of svo_ctx_bytes_in_use held by the pipeline lanes' workspaces,
  * which are kept between builds; the rest is trees and svo_device_alloc. */
 uint64_t svo_ctx_workspace_bytes(const svo_ctx* ctx);
+/* Device allocations (hipMalloc calls) the context has made: blocks freed by
+ * the library are cached and reused, so in a steady loop of builds of one
+ * shape this count stops growing after the first builds. */
+uint64_t svo_ctx_device_mallocs(const svo_ctx* ctx);
 /* Device memory on the context's device (counted against its budget), for
  * callers that keep meshes resident for svo_build_device / svo_build_batch. */
 int svo_device_alloc(svo_ctx* ctx, uint64_t bytes, void** out);
