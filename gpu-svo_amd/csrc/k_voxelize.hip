@@ -418,88 +418,130 @@ struct VoxOut {
 constexpr uint64_t BIG_ROWS = 512;
 
 // Voxelization order.  One lane walks each small triangle's box, so a wave
-// runs as long as the largest box among its 64 triangles.  k_classify sorts
-// each run of ORDER_GROUP consecutive triangles by box cells, largest first
-// (triangles past the one-lane limit, which k_vox_small only queues, then
-// eighth-octave classes, exact below 32 cells) into `order`: the waves of a
-// run see boxes of about one size.  Triangles of one class keep roughly their
+// runs as long as the largest box among its 64 triangles.  k_classify gives
+// every triangle a size class (one byte: triangles past the one-lane limit,
+// which k_vox_small only queues, first, then eighth-octave classes of the box
+// cells, exact below 32 cells, largest first), and k_order sorts each run of
+// ORDER_GROUP consecutive triangles by class into `order`: the waves of a run
+// see boxes of about one size.  Triangles of one class keep roughly their
 // input order (ranks are taken item by item), which keeps the neighbouring
 // triangles of a mesh together.  k_vox_small deals the runs' workgroup slots
 // out slot-major (the first slot of every run, then the second, ...), so the
 // workgroups of the largest boxes start first and the launch ends on the
 // smallest.  The order does not change the hits.
 constexpr uint32_t ORDER_THREADS = 1024;
-constexpr uint32_t ORDER_PER = 8;
+constexpr uint32_t ORDER_PER = 8;  // triangles per thread of k_order
 constexpr uint32_t ORDER_GROUP = ORDER_THREADS * ORDER_PER;
 constexpr uint32_t ORDER_BINS = 256;
 static_assert(ORDER_GROUP % BLOCK == 0, "a run fills whole workgroups of k_vox_small");
 
+// Size class, largest boxes first: 0 past the one-lane limit, then
+// 254 - (eighth-octave class of the cells); 255 for triangles that walk
+// nothing (failed mesh, bad index).
 __device__ inline uint32_t size_bin(uint64_t cells, uint64_t small_max) {
-  if (cells > small_max) return ORDER_BINS - 1;
-  if (cells < 32) return (uint32_t)cells;
+  if (cells > small_max) return 0;
+  if (cells < 32) return ORDER_BINS - 2 - (uint32_t)cells;
   const uint32_t e = 63u - (uint32_t)__builtin_clzll(cells);  // >= 5
   const uint32_t b = 32u + 8u * (e - 5u) + (uint32_t)((cells >> (e - 3u)) & 7u);
-  return b < ORDER_BINS - 2 ? b : ORDER_BINS - 2;
+  return ORDER_BINS - 2 - (b < ORDER_BINS - 3 ? b : ORDER_BINS - 3);
 }
 
-// Candidate cells and big-triangle chunk counts (totals[0], totals[1]) and
-// the voxelization order (one workgroup per run of ORDER_GROUP triangles).
-__global__ __launch_bounds__(ORDER_THREADS) void k_classify(const MeshDev* meshes,
-                                                            const MeshParams* params,
-                                                            const uint64_t* tri_base,
-                                                            uint32_t n_meshes, uint64_t n_tris,
-                                                            uint32_t small_max,
-                                                            unsigned long long* totals,
-                                                            uint32_t* order) {
-  __shared__ uint32_t s_bin[ORDER_BINS];
-  __shared__ uint64_t red[ORDER_THREADS / 64][2];
-  for (uint32_t i = threadIdx.x; i < ORDER_BINS; i += ORDER_THREADS) s_bin[i] = 0;
-  const uint64_t g0 = (uint64_t)blockIdx.x * ORDER_GROUP;
-  uint32_t bin[ORDER_PER];
+// Candidate cells and big-triangle chunk counts (added to totals[0],
+// totals[1]) and each triangle's size class (bins[gid]).  A grid-stride loop
+// over a bounded grid: the two totals take one atomic per workgroup, and
+// atomics on one address serialize.
+constexpr uint32_t CLASSIFY_BLOCKS = 1024;
+__global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
+                                                    const MeshParams* params,
+                                                    const uint64_t* tri_base, uint32_t n_meshes,
+                                                    uint64_t n_tris, uint32_t small_max,
+                                                    unsigned long long* totals, uint8_t* bins) {
   uint64_t cells = 0, chunks = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < ORDER_PER; ++k) {
-    const uint64_t gid = g0 + k * ORDER_THREADS + threadIdx.x;
-    bin[k] = ORDER_BINS - 1;  // a failed mesh or a bad index: nothing to walk, last
-    if (gid < n_tris) {
-      const uint32_t m = find_mesh(tri_base, n_meshes, gid);
-      const MeshDev& md = meshes[m];
-      const MeshParams& P = params[m];
-      float v[3][3];
-      if (P.status == 0 && load_tri(md, P, gid - md.tri_base, v)) {
-        int g0b[3], g1b[3];
-        tri_box(v, P, g0b, g1b);
-        const uint64_t nx = g1b[0] - g0b[0] + 1, ny = g1b[1] - g0b[1] + 1,
-                       nz = g1b[2] - g0b[2] + 1;
-        const uint64_t c = nx * ny * nz;
-        const uint64_t sm = md.small_max ? md.small_max : small_max;
-        cells += c;
-        if (c > sm) chunks += (ny * nz + BIG_ROWS - 1) / BIG_ROWS;
-        bin[k] = ORDER_BINS - 1 - size_bin(c, sm);  // largest first
-      }
+  for (uint64_t gid = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; gid < n_tris;
+       gid += (uint64_t)gridDim.x * BLOCK) {
+    uint32_t bin = ORDER_BINS - 1;
+    const uint32_t m = find_mesh(tri_base, n_meshes, gid);
+    const MeshDev& md = meshes[m];
+    const MeshParams& P = params[m];
+    float v[3][3];
+    if (P.status == 0 && load_tri(md, P, gid - md.tri_base, v)) {
+      int g0[3], g1[3];
+      tri_box(v, P, g0, g1);
+      const uint64_t nx = g1[0] - g0[0] + 1, ny = g1[1] - g0[1] + 1, nz = g1[2] - g0[2] + 1;
+      const uint64_t c = nx * ny * nz;
+      const uint64_t sm = md.small_max ? md.small_max : small_max;
+      cells += c;
+      if (c > sm) chunks += (ny * nz + BIG_ROWS - 1) / BIG_ROWS;
+      bin = size_bin(c, sm);
     }
+    bins[gid] = (uint8_t)bin;
   }
-  __syncthreads();  // s_bin cleared
-  uint32_t rank[ORDER_PER];
-#pragma unroll
-  for (uint32_t k = 0; k < ORDER_PER; ++k)
-    rank[k] = g0 + k * ORDER_THREADS + threadIdx.x < n_tris ? atomicAdd(&s_bin[bin[k]], 1u) : 0u;
   for (int off = 32; off > 0; off >>= 1) {
     cells += __shfl_xor(cells, off);
     chunks += __shfl_xor(chunks, off);
   }
+  __shared__ uint64_t red[WAVES][2];
   if ((threadIdx.x & 63) == 0) {
     red[threadIdx.x >> 6][0] = cells;
     red[threadIdx.x >> 6][1] = chunks;
   }
-  __syncthreads();  // bin counts and wave sums complete
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    uint64_t t = 0;
+    for (int w = 0; w < WAVES; ++w) t += red[w][threadIdx.x];
+    if (t) atomicAdd(totals + threadIdx.x, (unsigned long long)t);
+  }
+}
+
+// One workgroup per run of ORDER_GROUP triangles: a counting sort of the
+// run's size classes into `order`.  Each wave counts its triangles in a
+// histogram of its own (a run's triangles fall into few classes, and LDS
+// atomics on one address serialize across the whole workgroup), the
+// (class, wave) counts are scanned class-major, and every triangle lands at
+// its class's and wave's offset plus its rank in the wave's count.  Thread t
+// takes triangles t, t + 1024, ... of the run, so the lanes of a wave hold
+// consecutive triangles and each class keeps its triangles in input order
+// within a wave.
+constexpr uint32_t ORDER_WAVES = ORDER_THREADS / 64;
+__global__ __launch_bounds__(ORDER_THREADS) void k_order(const uint8_t* bins, uint64_t n_tris,
+                                                         uint32_t* order) {
+  __shared__ uint32_t s_cnt[ORDER_BINS][ORDER_WAVES];  // then (class, wave) offsets
+  __shared__ uint32_t s_tot[ORDER_BINS];
+  const uint32_t wave = threadIdx.x >> 6;
+  for (uint32_t i = threadIdx.x; i < ORDER_BINS * ORDER_WAVES; i += ORDER_THREADS)
+    (&s_cnt[0][0])[i] = 0;
+  __syncthreads();
+  const uint64_t g0 = (uint64_t)blockIdx.x * ORDER_GROUP;
+  uint32_t bin[ORDER_PER], rank[ORDER_PER];
+#pragma unroll
+  for (uint32_t k = 0; k < ORDER_PER; ++k) {
+    const uint64_t gid = g0 + k * ORDER_THREADS + threadIdx.x;
+    bin[k] = gid < n_tris ? bins[gid] : 0u;
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < ORDER_PER; ++k)
+    rank[k] = g0 + k * ORDER_THREADS + threadIdx.x < n_tris
+                  ? atomicAdd(&s_cnt[bin[k]][wave], 1u) : 0u;
+  __syncthreads();
+  // per class: exclusive prefix over the waves, total to s_tot
+  if (threadIdx.x < ORDER_BINS) {
+    uint32_t t = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < ORDER_WAVES; ++w) {
+      const uint32_t c = s_cnt[threadIdx.x][w];
+      s_cnt[threadIdx.x][w] = t;
+      t += c;
+    }
+    s_tot[threadIdx.x] = t;
+  }
+  __syncthreads();
   if (threadIdx.x < 64) {
-    // exclusive scan of the bin counts, four bins per lane
+    // exclusive scan of the class totals, four classes per lane
     const uint32_t l = threadIdx.x;
     uint32_t c[4], t = 0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      c[j] = s_bin[4 * l + j];
+      c[j] = s_tot[4 * l + j];
       t += c[j];
     }
     uint32_t inc = t;
@@ -510,20 +552,15 @@ __global__ __launch_bounds__(ORDER_THREADS) void k_classify(const MeshDev* meshe
     uint32_t ex = inc - t;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      s_bin[4 * l + j] = ex;
+      s_tot[4 * l + j] = ex;
       ex += c[j];
-    }
-    if (l < 2) {
-      uint64_t s = 0;
-      for (uint32_t w = 0; w < ORDER_THREADS / 64; ++w) s += red[w][l];
-      if (s) atomicAdd(totals + l, (unsigned long long)s);
     }
   }
   __syncthreads();
 #pragma unroll
   for (uint32_t k = 0; k < ORDER_PER; ++k) {
     const uint64_t gid = g0 + k * ORDER_THREADS + threadIdx.x;
-    if (gid < n_tris) order[g0 + s_bin[bin[k]] + rank[k]] = (uint32_t)gid;
+    if (gid < n_tris) order[g0 + s_tot[bin[k]] + s_cnt[bin[k]][wave] + rank[k]] = (uint32_t)gid;
   }
 }
 
@@ -710,14 +747,23 @@ void launch_params(const MeshDev* meshes, uint32_t n_meshes, const uint32_t* acc
 
 void launch_classify(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
                      uint32_t n_meshes, uint64_t n_tris, const Tuning& tn,
-                     unsigned long long* totals, uint32_t* order, hipStream_t s) {
+                     unsigned long long* totals, uint32_t* order, uint8_t* bins, hipStream_t s) {
   if (n_tris == 0) return;
   const uint32_t blocks = (uint32_t)((n_tris + BLOCK - 1) / BLOCK);
   hipLaunchKernelGGL(k_flag_bad_index, dim3(blocks), dim3(BLOCK), 0, s, meshes,
                      const_cast<MeshParams*>(params), tri_base, n_meshes, n_tris);
-  const uint32_t groups = (uint32_t)((n_tris + ORDER_GROUP - 1) / ORDER_GROUP);
-  hipLaunchKernelGGL(k_classify, dim3(groups), dim3(ORDER_THREADS), 0, s, meshes, params,
-                     tri_base, n_meshes, n_tris, tn.small_max_cells, totals, order);
+  hipLaunchKernelGGL(k_classify, dim3(std::min<uint32_t>(blocks, CLASSIFY_BLOCKS)), dim3(BLOCK), 0,
+                     s, meshes, params, tri_base, n_meshes, n_tris, tn.small_max_cells, totals,
+                     bins);
+  const uint32_t runs = (uint32_t)((n_tris + ORDER_GROUP - 1) / ORDER_GROUP);
+  hipLaunchKernelGGL(k_order, dim3(runs), dim3(ORDER_THREADS), 0, s, (const uint8_t*)bins, n_tris,
+                     order);
+}
+
+size_t order_bytes(uint64_t n_tris) {
+  // order (uint32 per triangle), then the size classes (one byte each)
+  const uint64_t n = std::max<uint64_t>(n_tris, 1);
+  return ((4 * n + 255) & ~(uint64_t)255) + ((n + 255) & ~(uint64_t)255);
 }
 
 template <typename K>

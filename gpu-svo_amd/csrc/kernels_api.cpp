@@ -172,7 +172,7 @@ int svo_k_voxelize(int device, const float* xyz, uint64_t n_points, const int32_
   uint32_t* acc = (uint32_t*)B.get(32);
   MeshParams* par = (MeshParams*)B.get(sizeof(MeshParams));
   unsigned long long* cnt = (unsigned long long*)B.get(64);
-  uint32_t* order = (uint32_t*)B.get(4 * n_tris);
+  uint32_t* order = (uint32_t*)B.get(order_bytes(n_tris));
   if (!dp || !dt || !md || !tb || !acc || !par || !cnt || !order) return SVO_E_OOM;
   MeshDev h{};
   h.pts = dp;
@@ -194,7 +194,8 @@ int svo_k_voxelize(int device, const float* xyz, uint64_t n_points, const int32_
       hipMemcpyAsync(cnt, zero, 64, hipMemcpyHostToDevice, s) == hipSuccess) {
     launch_bbox(md, 1, n_points, acc, s);
     launch_params(md, 1, acc, par, s);
-    launch_classify(md, par, tb, 1, n_tris, tn, cnt, order, s);
+    launch_classify(md, par, tb, 1, n_tris, tn, cnt, order,
+                    (uint8_t*)order + ((4 * n_tris + 255) & ~(uint64_t)255), s);
     if (hipMemcpyAsync(tot, cnt, 32, hipMemcpyDeviceToHost, s) == hipSuccess &&
         hipStreamSynchronize(s) == hipSuccess)
       st = SVO_OK;

@@ -411,7 +411,7 @@ uint64_t estimate_bytes(uint64_t n_points, uint64_t n_tris, const svo_params* p)
   const double per_node = (p->flags & SVO_BUILD_NO_VERTEX_EDGE) ? 170.0
                           : (p->semantics == SVO_SEM_INTENDED ? 800.0 : 640.0);
   const double b = nodes * per_node + hits * 8.0 * 2.5 + (double)n_points * 12.0 +
-                   (double)n_tris * 16.0 + (1 << 20);
+                   (double)n_tris * 12.0 + (double)order_bytes(n_tris) + (1 << 20);
   return (uint64_t)b;
 }
 
@@ -450,7 +450,8 @@ double pass_bytes_bound(const svo_params* p, int kb, uint64_t P, uint64_t T, int
     lv += K * cap + (l >= 2 ? 8.0 * cap : 0.0) + 3 * 256.0;
   }
   const double n_big = (double)T + rows / 512.0;
-  const double ws_common = 12.0 * P + 16.0 * T + 3 * 256.0 + lv + (double)level_tmp_bytes(kc) +
+  const double ws_common = 12.0 * P + 12.0 * T + (double)order_bytes(T) + 2 * 256.0 + lv +
+                           (double)level_tmp_bytes(kc) +
                            sizeof(BigItem) * std::max(n_big, 1.0) + (double)attr_tmp_bytes(4096) +
                            (1 << 20);  // control buffers of one mesh
   const double ws_bitmap = 4.0 * bm_words + (double)bitmap_level_tmp_bytes(bm_words);
@@ -1013,8 +1014,9 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   tn.small_max_cells = (uint32_t)std::min<uint64_t>(tn.small_max_cells,
                                                     std::max<uint64_t>(32, T / 128));
   uint32_t* order = nullptr;
-  SVO_TRY(W.get("order", 4 * std::max<uint64_t>(T, 1), (void**)&order));
-  launch_classify(d_md, d_par, d_tb, n, T, tn, d_cnt, order, s);
+  SVO_TRY(W.get("order", order_bytes(T), (void**)&order));
+  launch_classify(d_md, d_par, d_tb, n, T, tn, d_cnt, order,
+                  (uint8_t*)order + align256(4 * T), s);
   timer.mark("classify");
   uint64_t cells = 0, n_big = 0;
   if (capture) {

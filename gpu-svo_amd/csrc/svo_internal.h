@@ -54,10 +54,13 @@ void launch_params(const MeshDev* meshes, uint32_t n_meshes, const uint32_t* acc
 // Candidate cells and big-triangle chunk counts (2 x uint64 added to
 // `totals`), and the order launch_voxelize takes the triangles in
 // (`order`: n_tris uint32, a permutation of the triangle indices that
-// groups triangles of about one box size).
+// groups triangles of about one box size).  `order` heads a buffer of
+// order_bytes(n_tris) bytes whose tail `bins` (at byte 4 * n_tris rounded up
+// to 256) holds the triangles' size classes.
+size_t order_bytes(uint64_t n_tris);
 void launch_classify(const MeshDev* meshes, const MeshParams* params, const uint64_t* tri_base,
                      uint32_t n_meshes, uint64_t n_tris, const Tuning& tn,
-                     unsigned long long* totals, uint32_t* order, hipStream_t s);
+                     unsigned long long* totals, uint32_t* order, uint8_t* bins, hipStream_t s);
 // Surface voxelization (SVO.cu:37-188): emits composite keys.
 template <typename K>
 // Hits go to `keys` (raw, for the sort) when `bitmap` is null, else into the
