@@ -446,8 +446,13 @@ __device__ inline uint32_t size_bin(uint64_t cells, uint64_t small_max) {
   return ORDER_BINS - 2 - (b < ORDER_BINS - 3 ? b : ORDER_BINS - 3);
 }
 
+// A pass of at most one run keeps the triangles' own order: the run sort's
+// launch costs more there than it saves (the plain path's single-mesh passes
+// ran 10 % slower with it).
+inline bool sorted_order(uint64_t n_tris) { return n_tris > ORDER_GROUP; }
+
 // Candidate cells and big-triangle chunk counts (added to totals[0],
-// totals[1]) and each triangle's size class (bins[gid]).  A grid-stride loop
+// totals[1]) and each triangle's size class (bins[gid], when bins is set).  A grid-stride loop
 // over a bounded grid: the two totals take one atomic per workgroup, and
 // atomics on one address serialize.
 constexpr uint32_t CLASSIFY_BLOCKS = 1024;
@@ -474,7 +479,7 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
       if (c > sm) chunks += (ny * nz + BIG_ROWS - 1) / BIG_ROWS;
       bin = size_bin(c, sm);
     }
-    bins[gid] = (uint8_t)bin;
+    if (bins) bins[gid] = (uint8_t)bin;
   }
   for (int off = 32; off > 0; off >>= 1) {
     cells += __shfl_xor(cells, off);
@@ -593,7 +598,7 @@ __global__ __launch_bounds__(BLOCK) void k_vox_small(
   const uint32_t runs = (uint32_t)((n_tris + ORDER_GROUP - 1) / ORDER_GROUP);
   const uint64_t pos = (uint64_t)(blockIdx.x % runs) * ORDER_GROUP +
                        (uint64_t)(blockIdx.x / runs) * BLOCK + threadIdx.x;
-  const uint64_t gid = pos < n_tris ? order[pos] : n_tris;
+  const uint64_t gid = pos >= n_tris ? n_tris : order ? order[pos] : pos;
   TriSetup S;
   uint64_t count = 0;
   K mesh_key = 0;
@@ -754,7 +759,8 @@ void launch_classify(const MeshDev* meshes, const MeshParams* params, const uint
                      const_cast<MeshParams*>(params), tri_base, n_meshes, n_tris);
   hipLaunchKernelGGL(k_classify, dim3(std::min<uint32_t>(blocks, CLASSIFY_BLOCKS)), dim3(BLOCK), 0,
                      s, meshes, params, tri_base, n_meshes, n_tris, tn.small_max_cells, totals,
-                     bins);
+                     sorted_order(n_tris) ? bins : nullptr);
+  if (!sorted_order(n_tris)) return;
   const uint32_t runs = (uint32_t)((n_tris + ORDER_GROUP - 1) / ORDER_GROUP);
   hipLaunchKernelGGL(k_order, dim3(runs), dim3(ORDER_THREADS), 0, s, (const uint8_t*)bins, n_tris,
                      order);
@@ -774,6 +780,7 @@ void launch_voxelize(const MeshDev* meshes, const MeshParams* params, const uint
                      uint64_t big_cap, hipStream_t s) {
   if (n_tris == 0) return;
   const VoxOut<K> out{keys, bitmap, bm_off, key_count};
+  if (!sorted_order(n_tris)) order = nullptr;  // the triangles' own order
   // every run of ORDER_GROUP triangles has ORDER_GROUP / BLOCK workgroup slots
   const uint32_t blocks = (uint32_t)((n_tris + ORDER_GROUP - 1) / ORDER_GROUP) *
                           (ORDER_GROUP / BLOCK);
