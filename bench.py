@@ -506,6 +506,8 @@ def served_stream(n, n_warm, max_batch, lanes, max_wait_us):
         done = 0
         queries = 0
         t_start = time.perf_counter()
+        t_last = t_start
+        gap = [0.0, 0.0]  # longest time without a completion while requests were pending, and when
         i = 0
         while done < len(reqs):
             now = time.perf_counter() - t_start
@@ -513,7 +515,13 @@ def served_stream(n, n_warm, max_batch, lanes, max_wait_us):
                 _, P, T, res = reqs[i]
                 srv.submit(P, T, res)
                 i += 1
-            for d in srv.poll(cap=256, timeout_us=200 if i >= len(reqs) else 0):
+            got = srv.poll(cap=256, timeout_us=200 if i >= len(reqs) else 0)
+            t_now = time.perf_counter()
+            if got or i == done:
+                if got and t_now - t_last > gap[0]:
+                    gap[0], gap[1] = t_now - t_last, t_last - t_start
+                t_last = t_now
+            for d in got:
                 if d.status != 0:
                     raise RuntimeError(f"request {d.request_id} failed: {d.status}")
                 lat.append(d.t_done - d.t_submit)
@@ -525,15 +533,15 @@ def served_stream(n, n_warm, max_batch, lanes, max_wait_us):
                 queries += q.size
                 d.tree.free()
                 done += 1
-        return lat, bsz, wait, queries, time.perf_counter() - t_start
+        return lat, bsz, wait, queries, time.perf_counter() - t_start, gap
 
     cold_p99 = None
     if warm:
         log(f"bench: served warm-up stream of {len(warm)} requests (untimed)")
-        wl, _, _, _, _ = run(warm)
+        wl, _, _, _, _, _ = run(warm)
         cold_p99 = round(float(np.percentile(np.array(wl) * 1e3, 99)), 3)
     log(f"bench: served stream of {len(meshes)} requests")
-    lat, bsz, wait, queries, wall = run(meshes)
+    lat, bsz, wait, queries, wall, gap = run(meshes)
     srv.close()
     ctx.close()
     lat = np.array(lat) * 1e3
@@ -546,6 +554,8 @@ def served_stream(n, n_warm, max_batch, lanes, max_wait_us):
         "queue_wait_p50_ms": round(float(np.percentile(np.array(wait) * 1e3, 50)), 3),
         "queue_wait_p99_ms": round(float(np.percentile(np.array(wait) * 1e3, 99)), 3),
         "mean_batch": round(float(np.mean(bsz)), 2), "queries": queries,
+        "longest_completion_gap_ms": round(gap[0] * 1e3, 3),
+        "longest_gap_at_ms": round(gap[1] * 1e3, 3),
         "server": {"lanes": lanes, "max_batch": max_batch, "max_wait_us": max_wait_us,
                    "budget_gb": 64},
     }
