@@ -446,10 +446,11 @@ __device__ inline uint32_t size_bin(uint64_t cells, uint64_t small_max) {
   return ORDER_BINS - 2 - (b < ORDER_BINS - 3 ? b : ORDER_BINS - 3);
 }
 
-// A pass of at most one run keeps the triangles' own order: the run sort's
-// launch costs more there than it saves (the plain path's single-mesh passes
-// ran 10 % slower with it).
-inline bool sorted_order(uint64_t n_tris) { return n_tris > ORDER_GROUP; }
+// A pass of at most 8 runs (65,536 triangles) keeps the triangles' own
+// order: there the run sort's kernel (7-9 us, on the pass's critical path)
+// costs more than the voxelizer saves (256 single-mesh passes of 320-81,920
+// triangles took 80 ms sorted against 71-79 ms unsorted).
+inline bool sorted_order(uint64_t n_tris) { return n_tris > 8ull * ORDER_GROUP; }
 
 // Candidate cells and big-triangle chunk counts (added to totals[0],
 // totals[1]) and each triangle's size class (bins[gid], when bins is set).  A grid-stride loop
