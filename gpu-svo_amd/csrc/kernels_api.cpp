@@ -64,6 +64,50 @@ int noalloc_scratch(int dev, void** out) {
 
 }  // namespace
 
+namespace {
+template <typename K>
+int octree_level(const K* sorted, uint64_t n, int code_bits, K* unique, uint64_t* n_unique,
+                 K* parents, uint32_t* first_child, uint64_t* n_parents) {
+  // sorted, and every code below 2^code_bits (the span table has one row)
+  const K lim = (K)1 << code_bits;
+  for (uint64_t i = 0; i < n; ++i)
+    if (sorted[i] >= lim || (i && sorted[i] < sorted[i - 1])) return SVO_E_INVALID;
+  DevBufs B;
+  K* in = (K*)B.get(sizeof(K) * n);
+  K* e1 = (K*)B.get(sizeof(K) * n);
+  K* e2 = (K*)B.get(sizeof(K) * n);
+  uint32_t* fc = (uint32_t*)B.get(4 * n);
+  unsigned long long* cnt = (unsigned long long*)B.get(8 * 4);  // [n, U, G, -]
+  LevelSpan* spans = (LevelSpan*)B.get(sizeof(LevelSpan));
+  void* tmp = B.get(level_tmp_bytes(n));
+  if (!in || !e1 || !e2 || !fc || !cnt || !spans || !tmp) return SVO_E_OOM;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return SVO_E_HIP;
+  int st = SVO_E_HIP;
+  const unsigned long long hn[4] = {n, 0, 0, 0};
+  unsigned long long hc[4] = {0, 0, 0, 0};
+  if (hipMemcpyAsync(in, sorted, sizeof(K) * n, hipMemcpyHostToDevice, s) == hipSuccess &&
+      hipMemcpyAsync(cnt, hn, sizeof hn, hipMemcpyHostToDevice, s) == hipSuccess) {
+    const KeyLayout<K> kl{code_bits, (K)(lim - 1)};
+    const LevelArrays l1{e1, nullptr, nullptr, n}, l2{e2, nullptr, fc, n};
+    launch_level_first<K>(in, cnt, n, kl, l1, l2, cnt + 1, spans, 1, Tuning{}.scan_blocks, tmp, s);
+    if (hipMemcpyAsync(hc, cnt, sizeof hc, hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess && hipGetLastError() == hipSuccess &&
+        hc[1] <= n && hc[2] <= n &&
+        hipMemcpyAsync(unique, e1, sizeof(K) * hc[1], hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipMemcpyAsync(parents, e2, sizeof(K) * hc[2], hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipMemcpyAsync(first_child, fc, 4 * hc[2], hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess) {
+      *n_unique = hc[1];
+      *n_parents = hc[2];
+      st = SVO_OK;
+    }
+  }
+  (void)hipStreamDestroy(s);
+  return st;
+}
+}  // namespace
+
 extern "C" {
 
 int svo_k_radix_sort(int device, void* keys, uint64_t n, int key_bytes, int bits) {
@@ -103,6 +147,23 @@ int svo_k_radix_sort(int device, void* keys, uint64_t n, int key_bytes, int bits
   }
   (void)hipStreamDestroy(s);
   return st;
+}
+
+int svo_k_octree_level(int device, const void* sorted, uint64_t n, int key_bytes, int code_bits,
+                       void* unique, uint64_t* n_unique, void* parents, uint32_t* first_child,
+                       uint64_t* n_parents) {
+  if ((key_bytes != 4 && key_bytes != 8) || code_bits < 3 || code_bits >= 8 * key_bytes ||
+      !n_unique || !n_parents || n >= (1ull << 32) ||
+      (n && (!sorted || !unique || !parents || !first_child)))
+    return SVO_E_INVALID;
+  *n_unique = *n_parents = 0;
+  if (n == 0) return SVO_OK;
+  if (hipSetDevice(device) != hipSuccess) return SVO_E_HIP;
+  if (key_bytes == 4)
+    return octree_level<uint32_t>((const uint32_t*)sorted, n, code_bits, (uint32_t*)unique,
+                                  n_unique, (uint32_t*)parents, first_child, n_parents);
+  return octree_level<uint64_t>((const uint64_t*)sorted, n, code_bits, (uint64_t*)unique,
+                                n_unique, (uint64_t*)parents, first_child, n_parents);
 }
 
 int svo_k_radix_sort_noalloc(int device, void* keys, uint64_t n, int key_bytes, int bits,

@@ -153,6 +153,9 @@ _SIGS = {
     "svo_k_radix_sort": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_int, C.c_int]),
     "svo_k_radix_sort_noalloc": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_int, C.c_int,
                                            C.c_void_p]),
+    "svo_k_octree_level": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_int, C.c_int,
+                                     C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_void_p,
+                                     C.POINTER(C.c_uint64)]),
     "svo_k_voxelize": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
                                  C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint64,
                                  C.POINTER(C.c_uint64)]),
@@ -273,6 +276,22 @@ def k_radix_sort(keys: np.ndarray, bits: int | None = None, device: int = 0) -> 
     _check(lib().svo_k_radix_sort(device, a.ctypes.data, a.size, a.dtype.itemsize, nb),
            "radix sort")
     return a
+
+
+def k_octree_level(sorted_codes: np.ndarray, code_bits: int, device: int = 0):
+    """One level step on sorted codes (svo_k_octree_level): (unique codes,
+    parent codes, index of each parent group's first member in unique)."""
+    a = np.ascontiguousarray(sorted_codes)
+    assert a.dtype in (np.uint32, np.uint64)
+    n = a.size
+    uniq = np.empty(max(n, 1), a.dtype)
+    par = np.empty(max(n, 1), a.dtype)
+    fc = np.empty(max(n, 1), np.uint32)
+    nu, npar = C.c_uint64(), C.c_uint64()
+    _check(lib().svo_k_octree_level(device, a.ctypes.data if n else None, n, a.dtype.itemsize,
+                                    code_bits, uniq.ctypes.data, C.byref(nu), par.ctypes.data,
+                                    fc.ctypes.data, C.byref(npar)), "octree level")
+    return uniq[:nu.value].copy(), par[:npar.value].copy(), fc[:npar.value].copy()
 
 
 def k_voxelize(points, tris, res, key_bits=32, device: int = 0):

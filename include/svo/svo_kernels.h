@@ -10,6 +10,11 @@
  *                       (SVO.cu:367-405, 208-217).
  *   svo_k_voxelize      surface voxelization of one mesh (surfaceVoxelize,
  *                       SVO.cu:37-188): the raw hit codes.
+ *   svo_k_octree_level  one level step on sorted codes: the unique codes
+ *                       (scanMortonFlag + exclusive_scan + compactArray,
+ *                       SVO.cu:30-35, 386-391, 208-217) and their parent
+ *                       groups (getParentMorton + cpNumNodes, SVO.cu:22-24,
+ *                       221-235, 426-437).
  */
 #ifndef SVO_SVO_KERNELS_H
 #define SVO_SVO_KERNELS_H
@@ -36,6 +41,19 @@ int svo_k_radix_sort(int device, void* keys, uint64_t n, int key_bytes, int bits
  * current device is restored on return. */
 int svo_k_radix_sort_noalloc(int device, void* keys, uint64_t n, int key_bytes, int bits,
                              void* stream);
+
+/* One octree level step on n sorted codes (key_bytes 4 or 8, duplicates
+ * allowed, every code below 2^code_bits, 3 <= code_bits < 8 * key_bytes):
+ * writes the n_unique distinct codes in order to `unique`, and for each of
+ * the n_parents parent groups (codes 8p..8p+7 share parent p = code >> 3)
+ * the parent code to `parents` and the index in `unique` of the group's
+ * first member to `first_child`.  Each output holds up to n entries.
+ * Unsorted input or a code of 2^code_bits or more is SVO_E_INVALID.  The
+ * parents of one level are the sorted codes of the next: calling it again
+ * on `parents` builds the level above. */
+int svo_k_octree_level(int device, const void* sorted, uint64_t n, int key_bytes, int code_bits,
+                       void* unique, uint64_t* n_unique, void* parents, uint32_t* first_child,
+                       uint64_t* n_parents);
 
 /* Voxelizes one mesh: writes up to cap hit codes (uint32 for key_bits 32,
  * uint64 for 64) in emission order, duplicates included; n_hits receives the

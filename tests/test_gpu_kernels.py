@@ -3,7 +3,10 @@
 svo_k_radix_sort is checked against numpy's stable sort on the same bits
 (tile edges 4095/4096/4097, runs of equal keys, presorted and reversed input,
 partial-bit sorts whose stability shows in the high bits); svo_k_voxelize's
-raw hits, deduplicated, against the oracle's voxel set.
+raw hits, deduplicated, against the oracle's voxel set; svo_k_octree_level
+against numpy's unique on random sorted codes with duplicates, and chained
+from the oracle's voxels up to the root against the oracle's node codes of
+every level.
 """
 import numpy as np
 import pytest
@@ -78,3 +81,60 @@ def test_voxelize_matches_oracle(bunny, res, kb):
     st, ref = po.voxelize(P, T, res, key_bits=kb)
     assert st == 0
     assert np.array_equal(np.unique(hits.astype(np.uint64)), np.unique(ref))
+
+
+def _level_expect(keys):
+    uniq = np.unique(keys)
+    par_all = uniq >> uniq.dtype.type(3)
+    parents, first = np.unique(par_all, return_index=True)
+    return uniq, parents, first.astype(np.uint32)
+
+
+@pytest.mark.parametrize("n,dt,bits", [(1, np.uint32, 30), (5000, np.uint32, 30),
+                                       (1_000_003, np.uint32, 30), (300_000, np.uint64, 45),
+                                       (300_000, np.uint64, 63)])
+def test_octree_level_random(n, dt, bits):
+    rng = np.random.default_rng(n + bits)
+    # duplicates and dense runs: draw from a range about n wide, then sort
+    lo = int(rng.integers(0, (1 << bits) - 4 * n))
+    keys = np.sort((lo + rng.integers(0, 4 * n, size=n)).astype(dt))
+    print(f"octree level case: n {n} dtype {np.dtype(dt).name} code bits {bits}", flush=True)
+    uniq, par, fc = svo_amd.k_octree_level(keys, bits)
+    eu, ep, ef = _level_expect(keys)
+    assert uniq.dtype == keys.dtype and par.dtype == keys.dtype
+    assert np.array_equal(uniq, eu)
+    assert np.array_equal(par, ep)
+    assert np.array_equal(fc, ef)
+
+
+def test_octree_level_chain_matches_oracle_levels(bunny):
+    """Level steps from the oracle's voxels up to the root give the parent codes
+    of the oracle's tree, level by level (its level-l nodes are the eight
+    children 8p..8p+7 of every level-(l+1) parent p)."""
+    P, T = bunny
+    ref = po.build(P, T, 128, key_bits=32, fields={"voxels", "code", "level_nodes"})
+    assert ref.status == 0
+    codes, counts = ref.arrays["code"], ref.arrays["level_nodes"].astype(np.int64)
+    cur = ref.arrays["voxels"]
+    off = 0
+    for l, nl in enumerate(counts[:-1]):
+        print(f"octree level chain: level {l + 1}, {cur.size} codes", flush=True)
+        uniq, par, fc = svo_amd.k_octree_level(cur, 30)
+        assert np.array_equal(uniq, cur)  # already unique
+        level_codes = codes[off:off + nl]
+        assert np.array_equal(np.unique(level_codes >> np.uint32(3)), par)
+        assert np.array_equal(fc, np.searchsorted(uniq >> np.uint32(3), par).astype(np.uint32))
+        off += nl
+        cur = par
+    assert cur.size == 1
+
+
+def test_octree_level_rejects_bad_input():
+    with pytest.raises(svo_amd.SvoError):
+        svo_amd.k_octree_level(np.array([5, 3], dtype=np.uint32), 30)  # unsorted
+    with pytest.raises(svo_amd.SvoError):
+        svo_amd.k_octree_level(np.array([1, 1 << 30], dtype=np.uint32), 30)  # past code_bits
+    with pytest.raises(svo_amd.SvoError):
+        svo_amd.k_octree_level(np.array([1], dtype=np.uint32), 32)  # code_bits too large
+    uniq, par, fc = svo_amd.k_octree_level(np.empty(0, dtype=np.uint64), 40)
+    assert uniq.size == par.size == fc.size == 0
