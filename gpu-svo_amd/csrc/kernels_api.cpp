@@ -166,6 +166,44 @@ int svo_k_octree_level(int device, const void* sorted, uint64_t n, int key_bytes
                                 n_unique, (uint64_t*)parents, first_child, n_parents);
 }
 
+int svo_k_ref_vertex_edge(int device, const float* origin, const float* width, uint64_t n_nodes,
+                          void* vertices, void* edges) {
+  if (n_nodes >= (1ull << 32) || (n_nodes && (!origin || !width || !vertices || !edges)))
+    return SVO_E_INVALID;
+  if (n_nodes == 0) return SVO_OK;
+  if (hipSetDevice(device) != hipSuccess) return SVO_E_HIP;
+  DevBufs B;
+  float* d_o = (float*)B.get(12 * n_nodes);
+  float* d_w = (float*)B.get(4 * n_nodes);
+  void* d_v = B.get(sizeof(svo_vertex) * 8 * n_nodes);
+  void* d_e = B.get(sizeof(svo_edge) * 12 * n_nodes);
+  uint64_t* d_base = (uint64_t*)B.get(16);
+  unsigned long long* d_tot = (unsigned long long*)B.get(64);
+  if (!d_o || !d_w || !d_v || !d_e || !d_base || !d_tot) return SVO_E_OOM;
+  hipStream_t s;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return SVO_E_HIP;
+  int st = SVO_E_HIP;
+  const uint64_t base[2] = {0, n_nodes};  // one mesh: owners are node indices
+  if (hipMemcpyAsync(d_o, origin, 12 * n_nodes, hipMemcpyHostToDevice, s) == hipSuccess &&
+      hipMemcpyAsync(d_w, width, 4 * n_nodes, hipMemcpyHostToDevice, s) == hipSuccess &&
+      hipMemcpyAsync(d_base, base, 16, hipMemcpyHostToDevice, s) == hipSuccess) {
+    NodeArrays na{};
+    na.origin = d_o;
+    na.width = d_w;
+    AttrOut out{(svo_vertex*)d_v, (svo_edge*)d_e, nullptr, d_tot};
+    launch_ref_attrs(na, n_nodes, d_base, 1, out, s);
+    if (hipGetLastError() == hipSuccess &&
+        hipMemcpyAsync(vertices, d_v, sizeof(svo_vertex) * 8 * n_nodes, hipMemcpyDeviceToHost,
+                       s) == hipSuccess &&
+        hipMemcpyAsync(edges, d_e, sizeof(svo_edge) * 12 * n_nodes, hipMemcpyDeviceToHost, s) ==
+            hipSuccess &&
+        hipStreamSynchronize(s) == hipSuccess)
+      st = SVO_OK;
+  }
+  (void)hipStreamDestroy(s);
+  return st;
+}
+
 int svo_k_radix_sort_noalloc(int device, void* keys, uint64_t n, int key_bytes, int bits,
                              void* stream) {
   if ((key_bytes != 4 && key_bytes != 8) || bits < 0 || bits > 8 * key_bytes || (n && !keys))

@@ -153,6 +153,8 @@ _SIGS = {
     "svo_k_radix_sort": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_int, C.c_int]),
     "svo_k_radix_sort_noalloc": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_int, C.c_int,
                                            C.c_void_p]),
+    "svo_k_ref_vertex_edge": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_uint64,
+                                        C.c_void_p, C.c_void_p]),
     "svo_k_octree_level": (C.c_int, [C.c_int, C.c_void_p, C.c_uint64, C.c_int, C.c_int,
                                      C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p, C.c_void_p,
                                      C.POINTER(C.c_uint64)]),
@@ -276,6 +278,22 @@ def k_radix_sort(keys: np.ndarray, bits: int | None = None, device: int = 0) -> 
     _check(lib().svo_k_radix_sort(device, a.ctypes.data, a.size, a.dtype.itemsize, nb),
            "radix sort")
     return a
+
+
+def k_ref_vertex_edge(origin: np.ndarray, width: np.ndarray, device: int = 0):
+    """Reference-semantics vertex and edge records of nodes
+    (svo_k_ref_vertex_edge): (vertices [8n], edges [12n]) structured arrays."""
+    o = np.ascontiguousarray(origin, np.float32).reshape(-1, 3)
+    w = np.ascontiguousarray(width, np.float32).reshape(-1)
+    assert o.shape[0] == w.size
+    n = w.size
+    verts = np.empty(8 * n, VERT_DTYPE)
+    edges = np.empty(12 * n, EDGE_DTYPE)
+    _check(lib().svo_k_ref_vertex_edge(device, o.ctypes.data if n else None,
+                                       w.ctypes.data if n else None, n,
+                                       verts.ctypes.data if n else None,
+                                       edges.ctypes.data if n else None), "vertex/edge records")
+    return verts, edges
 
 
 def k_octree_level(sorted_codes: np.ndarray, code_bits: int, device: int = 0):

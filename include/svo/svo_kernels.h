@@ -10,6 +10,9 @@
  *                       (SVO.cu:367-405, 208-217).
  *   svo_k_voxelize      surface voxelization of one mesh (surfaceVoxelize,
  *                       SVO.cu:37-188): the raw hit codes.
+ *   svo_k_ref_vertex_edge  vertex and edge records of nodes, reference
+ *                       semantics (determineNodeVertex, determineNodeEdge,
+ *                       SVO.cu:567-677).
  *   svo_k_octree_level  one level step on sorted codes: the unique codes
  *                       (scanMortonFlag + exclusive_scan + compactArray,
  *                       SVO.cu:30-35, 386-391, 208-217) and their parent
@@ -54,6 +57,17 @@ int svo_k_radix_sort_noalloc(int device, void* keys, uint64_t n, int key_bytes, 
 int svo_k_octree_level(int device, const void* sorted, uint64_t n, int key_bytes, int code_bits,
                        void* unique, uint64_t* n_unique, void* parents, uint32_t* first_child,
                        uint64_t* n_parents);
+
+/* Vertex and edge records of n_nodes nodes as the reference builds them
+ * (SVO.cu:567-677): with the reference's neighbor table (every slot empty
+ * but the root's own) the owner of every record is its node.  Node i (origin
+ * 3 floats, width 1 float) gets 8 vertex records at vertices[8i..8i+7]
+ * (16 bytes each, svo_vertex in svo.h: the corner offset
+ * w * (j & 1, j >> 1 & 1, j >> 2 & 1), owner i) and 12 edge records at
+ * edges[12i..12i+11] (28 bytes each, svo_edge: both endpoints, absolute,
+ * in the reference's edge order, owner i).  Host buffers. */
+int svo_k_ref_vertex_edge(int device, const float* origin, const float* width, uint64_t n_nodes,
+                          void* vertices, void* edges);
 
 /* Voxelizes one mesh: writes up to cap hit codes (uint32 for key_bits 32,
  * uint64 for 64) in emission order, duplicates included; n_hits receives the

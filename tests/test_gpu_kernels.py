@@ -6,7 +6,8 @@ partial-bit sorts whose stability shows in the high bits); svo_k_voxelize's
 raw hits, deduplicated, against the oracle's voxel set; svo_k_octree_level
 against numpy's unique on random sorted codes with duplicates, and chained
 from the oracle's voxels up to the root against the oracle's node codes of
-every level.
+every level; svo_k_ref_vertex_edge on the oracle's node origins and widths
+against the oracle's vertex and edge records.
 """
 import numpy as np
 import pytest
@@ -138,3 +139,25 @@ def test_octree_level_rejects_bad_input():
         svo_amd.k_octree_level(np.array([1], dtype=np.uint32), 32)  # code_bits too large
     uniq, par, fc = svo_amd.k_octree_level(np.empty(0, dtype=np.uint64), 40)
     assert uniq.size == par.size == fc.size == 0
+
+
+@pytest.mark.parametrize("res", [64, 100])
+def test_ref_vertex_edge_matches_oracle(bunny, res):
+    """The oracle tree's node origins and widths through svo_k_ref_vertex_edge
+    give the oracle's vertex and edge records bit for bit (reference
+    semantics, where the consecutive unique removes nothing on the bunny)."""
+    P, T = bunny
+    ref = po.build(P, T, res, key_bits=32, fields={"origin", "width", "vertices", "edges"})
+    assert ref.status == 0
+    o, w = ref.arrays["origin"], ref.arrays["width"]
+    print(f"vertex/edge records case: bunny res {res}, {w.size} nodes", flush=True)
+    verts, edges = svo_amd.k_ref_vertex_edge(o, w)
+    assert verts.size == ref.arrays["vertices"].size == 8 * w.size
+    assert edges.size == ref.arrays["edges"].size == 12 * w.size
+    assert np.array_equal(verts.view(np.uint32), ref.arrays["vertices"].view(np.uint32))
+    assert np.array_equal(edges.view(np.uint32), ref.arrays["edges"].view(np.uint32))
+
+
+def test_ref_vertex_edge_empty():
+    v, e = svo_amd.k_ref_vertex_edge(np.empty((0, 3), np.float32), np.empty(0, np.float32))
+    assert v.size == 0 and e.size == 0
