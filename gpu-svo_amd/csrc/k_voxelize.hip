@@ -424,8 +424,7 @@ constexpr uint64_t BIG_ROWS = 512;
 // cells, exact below 32 cells, largest first), and k_order sorts each run of
 // ORDER_GROUP consecutive triangles by class into `order`: the waves of a run
 // see boxes of about one size.  Triangles of one class keep roughly their
-// input order (ranks are taken item by item), which keeps the neighbouring
-// triangles of a mesh together.  k_vox_small deals the runs' workgroup slots
+// input order, which keeps the neighbouring triangles of a mesh together.  k_vox_small deals the runs' workgroup slots
 // out slot-major (the first slot of every run, then the second, ...), so the
 // workgroups of the largest boxes start first and the launch ends on the
 // smallest.  The order does not change the hits.
@@ -453,9 +452,9 @@ __device__ inline uint32_t size_bin(uint64_t cells, uint64_t small_max) {
 inline bool sorted_order(uint64_t n_tris) { return n_tris > 8ull * ORDER_GROUP; }
 
 // Candidate cells and big-triangle chunk counts (added to totals[0],
-// totals[1]) and each triangle's size class (bins[gid], when bins is set).  A grid-stride loop
-// over a bounded grid: the two totals take one atomic per workgroup, and
-// atomics on one address serialize.
+// totals[1]) and each triangle's size class (bins[gid], when bins is set).
+// A grid-stride loop over a bounded grid: the two totals take one atomic per
+// workgroup, and atomics on one address serialize.
 constexpr uint32_t CLASSIFY_BLOCKS = 1024;
 __global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
                                                     const MeshParams* params,
@@ -506,8 +505,8 @@ __global__ __launch_bounds__(BLOCK) void k_classify(const MeshDev* meshes,
 // (class, wave) counts are scanned class-major, and every triangle lands at
 // its class's and wave's offset plus its rank in the wave's count.  Thread t
 // takes triangles t, t + 1024, ... of the run, so the lanes of a wave hold
-// consecutive triangles and each class keeps its triangles in input order
-// within a wave.
+// consecutive triangles and a class's triangles from one wave land near each
+// other (the order of ranks within one atomic instruction is the hardware's).
 constexpr uint32_t ORDER_WAVES = ORDER_THREADS / 64;
 __global__ __launch_bounds__(ORDER_THREADS) void k_order(const uint8_t* bins, uint64_t n_tris,
                                                          uint32_t* order) {
