@@ -176,3 +176,37 @@ def test_autotune_keeps_results_and_round_trips(gpu_ctx, tmp_path):
             b.free()
     finally:
         ctx.close()
+
+
+def test_sorted_batch_with_device_flagged_mesh(gpu_ctx):
+    """A device-resident batch of more than 65,536 triangles (voxelized in
+    size-sorted order) holding a mesh with an out-of-range index: that mesh
+    fails with E_INVALID (found on the device, where its triangles are
+    classed to walk nothing), every other mesh equals its plain build."""
+    import svo_amd
+    meshes = _meshes(36, seed=25, res_choices=(64, 128))
+    assert sum(T.shape[0] for _, T, _ in meshes) > 65536
+    k = 5
+    P, T, _ = meshes[k]
+    t_bad = T.copy()
+    t_bad[T.shape[0] // 2, 1] = P.shape[0] + 7
+    dms = [gpu_ctx.upload_mesh(P_, t_bad if i == k else T_)
+           for i, (P_, T_, _) in enumerate(meshes)]
+    try:
+        print(f"sorted batch of {len(meshes)} device meshes, mesh {k} with a bad index",
+              flush=True)
+        trees, st = gpu_ctx.build_batch([d.desc(r) for d, (_, _, r) in zip(dms, meshes)],
+                                        on_device=True)
+        assert st[k] == svo_amd.E_INVALID and trees[k] is None
+        assert all(s == 0 for i, s in enumerate(st) if i != k), st
+        for i, ((P_, T_, res), tb) in enumerate(zip(meshes, trees)):
+            if i == k:
+                continue
+            tp = gpu_ctx.build(P_, T_, res)
+            for f in ("voxels", "code", "origin", "parent", "childs", "vertices", "edges"):
+                assert_same(f"mesh {i} {f}", tb.field(f), tp.field(f))
+            tp.free()
+            tb.free()
+    finally:
+        for d in dms:
+            d.free()
