@@ -405,14 +405,17 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
   __shared__ uint64_t s_moff[2];  // word range [s_moff[0], s_moff[1]) of mesh s_mesh
   const uint64_t tw0 = (uint64_t)blockIdx.x * BM_TILE;
   const uint32_t l0 = threadIdx.x * BM_WPT;
+  // the bitmap words and the tile's output offsets are loaded first, so that
+  // the mesh lookup's dependent loads run while they are in flight
+  uint32_t w[BM_WPT];
+  load_words(bits, n_words, tw0 + l0, w);
+  const uint32_t bu = tile_u[blockIdx.x], bg = tile_g[blockIdx.x];
   if (threadIdx.x == 0) {
     const uint32_t m = mesh_of_word(bm_off, n_meshes, tw0);
     s_mesh = m;
     s_moff[0] = bm_off[m];
     s_moff[1] = bm_off[m + 1];
   }
-  uint32_t w[BM_WPT];
-  load_words(bits, n_words, tw0 + l0, w);
   clear_words(bits, n_words, tw0 + l0, w);  // the bitmap is left clear for the next pass
   // packed counts: set bits in 0-23, nonzero bytes in 24-43, nonzero words in 44-63
   uint64_t cnt = 0;
@@ -440,7 +443,6 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
   __syncthreads();
   const uint32_t nz = (uint32_t)(tot >> 44);
   const uint32_t pre_end = (uint32_t)(tot & 0xFFFFFFu) | ((uint32_t)((tot >> 24) & 0xFFFFFu) << 17);
-  const uint32_t bu = tile_u[blockIdx.x], bg = tile_g[blockIdx.x];
   const uint32_t m0 = s_mesh;
   const uint64_t moff0 = s_moff[0], moff1 = s_moff[1];
   for (uint32_t r0 = 0; r0 < nz; r0 += BLOCK) {
