@@ -396,7 +396,7 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
                                                    KeyLayout<K> kl, const uint32_t* tile_u,
                                                    const uint32_t* tile_g, K* elem1, K* elem2,
                                                    uint32_t* first_child2) {
-  __shared__ uint64_t tmp[4];
+  __shared__ uint32_t tmp[8];
   __shared__ uint32_t s_word[BM_TILE], s_pre[BM_TILE];
   __shared__ uint16_t s_idx[BM_TILE];
   __shared__ K s_key[EMIT_CAP], s_par[BLOCK * 4];
@@ -417,17 +417,17 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
     s_moff[1] = bm_off[m + 1];
   }
   clear_words(bits, n_words, tw0 + l0, w);  // the bitmap is left clear for the next pass
-  // packed counts: set bits in 0-23, nonzero bytes in 24-43, nonzero words in 44-63
-  uint64_t cnt = 0;
+  // a: set bits (0-16) and nonzero words (17-28); b: nonzero bytes
+  uint32_t a = 0, b = 0;
 #pragma unroll
-  for (int j = 0; j < BM_WPT; ++j)
-    cnt += (uint64_t)__popc(w[j]) | ((uint64_t)nonzero_bytes(w[j]) << 24) |
-           ((uint64_t)(w[j] != 0) << 44);
-  uint64_t tot;
-  const uint64_t ex = block_excl_scan256(cnt, tmp, tot);
+  for (int j = 0; j < BM_WPT; ++j) {
+    a += __popc(w[j]) + ((uint32_t)(w[j] != 0) << 17);
+    b += nonzero_bytes(w[j]);
+  }
+  uint32_t ta, tb;
+  block_excl_scan256_2(a, b, tmp, ta, tb);  // tmp is not used again
   {
-    uint32_t pu = (uint32_t)(ex & 0xFFFFFFu), pg = (uint32_t)((ex >> 24) & 0xFFFFFu),
-             pz = (uint32_t)(ex >> 44);
+    uint32_t pu = a & 0x1FFFFu, pg = b, pz = a >> 17;
 #pragma unroll
     for (int j = 0; j < BM_WPT; ++j) {
       if (w[j]) {
@@ -441,8 +441,8 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
     }
   }
   __syncthreads();
-  const uint32_t nz = (uint32_t)(tot >> 44);
-  const uint32_t pre_end = (uint32_t)(tot & 0xFFFFFFu) | ((uint32_t)((tot >> 24) & 0xFFFFFu) << 17);
+  const uint32_t nz = ta >> 17;
+  const uint32_t pre_end = (ta & 0x1FFFFu) | (tb << 17);
   const uint32_t m0 = s_mesh;
   const uint64_t moff0 = s_moff[0], moff1 = s_moff[1];
   for (uint32_t r0 = 0; r0 < nz; r0 += BLOCK) {

@@ -43,6 +43,51 @@ __device__ inline T block_excl_scan256(T v, T* tmp, T& total) {
   return pre + inc - v;
 }
 
+// Inclusive scan of a 32-bit value over the wave through DPP lane moves (no
+// LDS traffic): shifts by 1, 2, 4 and 8 lanes inside each row of 16, then
+// lane 15 of each row added into the next row, then lane 31 into rows 2-3.
+// Lanes that a move does not reach receive 0.
+__device__ inline uint32_t wave_incl_scan_dpp(uint32_t v) {
+  int x = (int)v;
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, true);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, true);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, true);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, true);  // row_shr:8
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);  // row_bcast:15, rows 1, 3
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xC, 0xF, false);  // row_bcast:31, rows 2, 3
+  return (uint32_t)x;
+}
+
+// Exclusive scan of two 32-bit values over a 256-thread block, in place;
+// ta / tb receive the block sums.  `tmp` holds 8 values.  One barrier: a
+// caller that scans again with the same `tmp` must put a barrier between the
+// calls.
+__device__ inline void block_excl_scan256_2(uint32_t& a, uint32_t& b, uint32_t* tmp,
+                                            uint32_t& ta, uint32_t& tb) {
+  const uint32_t w = threadIdx.x >> 6;
+  const uint32_t ia = wave_incl_scan_dpp(a), ib = wave_incl_scan_dpp(b);
+  if ((threadIdx.x & 63) == 63) {
+    tmp[w] = ia;
+    tmp[4 + w] = ib;
+  }
+  __syncthreads();
+  uint32_t pa = 0, pb = 0, sa = 0, sb = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t xa = tmp[k], xb = tmp[4 + k];
+    if (k < w) {
+      pa += xa;
+      pb += xb;
+    }
+    sa += xa;
+    sb += xb;
+  }
+  ta = sa;
+  tb = sb;
+  a = pa + ia - a;
+  b = pb + ib - b;
+}
+
 // Three-component exclusive scan (vertex, edge, face counts).
 struct U64x3 {
   uint64_t a, b, c;
