@@ -484,14 +484,21 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
       }
       // groups: the nonzero bytes in order; first_child2 is the unique key of
       // the byte's lowest set bit
+      // the four bytes are tested in turn (a loop over the nonzero bytes runs
+      // as often as the wave's fullest word has them, nearly always four);
+      // the parent of byte b's keys is parent(hi) + b, and s_fc holds the
+      // tile-relative index (the set bits below the byte's lowest are those
+      // of the lower bytes)
       uint32_t lg = (pre >> 17) - g0;
-      const uint32_t t = (word & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;
-      // (s_fc holds the tile-relative index; the set bits below the byte's
-      // lowest are those of the lower bytes)
-      for (uint32_t nzb = (t | word) & 0x80808080u; nzb; nzb &= nzb - 1u, ++lg) {
-        const uint32_t sh = (uint32_t)__builtin_ctz(nzb) - 7u;  // 8 x byte index
-        s_par[lg] = kl.parent(hi | (K)sh);
-        s_fc[lg] = (pre & 0x1FFFFu) + __popc(word & ((1u << sh) - 1u));
+      const K phi = kl.parent(hi);
+      const uint32_t pu_w = pre & 0x1FFFFu;
+#pragma unroll
+      for (uint32_t b = 0; b < 4; ++b) {
+        if ((word >> (8 * b)) & 0xFFu) {
+          s_par[lg] = phi | (K)b;
+          s_fc[lg] = pu_w + (b ? __popc(word & ((1u << (8 * b)) - 1u)) : 0u);
+          ++lg;
+        }
       }
     }
     __syncthreads();
