@@ -79,7 +79,7 @@ def stage_bytes(stats, name):
 # x 2, WRITE_SIZE x 1; profiles/batch256_hbm_traffic.md).  Used to fill
 # roofline.traffic for this run's launch.
 TRAFFIC_PER_ALG_BYTE = {"attr_write": 1.000, "neighbor_fill": 1.000, "nodes": 1.00,
-                        "unique_level1": 1.00, "voxelize": 2.52}
+                        "unique_level1": 1.00, "voxelize": 4.2}
 
 
 def dedup_gbs(stats, stages):
