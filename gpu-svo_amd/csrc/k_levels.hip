@@ -384,9 +384,13 @@ __device__ inline uint32_t mesh_of_word(const uint64_t* bm_off, uint32_t n_meshe
 // nonzero bytes (groups) into LDS buffers, which the block writes out
 // contiguously.  A lane per nonzero word keeps most lanes busy (a lane per
 // bit would leave about four in five idle at surface densities), and staging
-// turns the lanes' strided stores into whole cache lines.  A round with more
-// than EMIT_CAP keys (dense tiles) stores its keys directly.  The tile's
-// words are zeroed after they are read.
+// turns the lanes' strided stores into whole cache lines.  A group is staged
+// as the 16-bit tile index of its first key only; its parent key is read
+// back from that key at copy-out.  That keeps the workgroup's LDS near 30 KB
+// with 32-bit keys, so five workgroups fit a CU where four did with the
+// parents staged.  A round with more than EMIT_CAP keys (dense tiles) stores
+// its keys and groups directly.  The tile's words are zeroed after they are
+// read.
 constexpr uint32_t EMIT_CAP = 2048;
 static_assert((EMIT_CAP & (EMIT_CAP - 1)) == 0, "the copy-out masks slot indices");
 
@@ -399,8 +403,8 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
   __shared__ uint32_t tmp[8];
   __shared__ uint32_t s_word[BM_TILE], s_pre[BM_TILE];
   __shared__ uint16_t s_idx[BM_TILE];
-  __shared__ K s_key[EMIT_CAP], s_par[BLOCK * 4];
-  __shared__ uint32_t s_fc[BLOCK * 4];
+  __shared__ K s_key[EMIT_CAP];
+  __shared__ uint16_t s_fc[BLOCK * 4];
   __shared__ uint32_t s_mesh;
   __shared__ uint64_t s_moff[2];  // word range [s_moff[0], s_moff[1]) of mesh s_mesh
   const uint64_t tw0 = (uint64_t)blockIdx.x * BM_TILE;
@@ -490,14 +494,26 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
       // tile-relative index (the set bits below the byte's lowest are those
       // of the lower bytes)
       uint32_t lg = (pre >> 17) - g0;
-      const K phi = kl.parent(hi);
       const uint32_t pu_w = pre & 0x1FFFFu;
+      if (stage) {
 #pragma unroll
-      for (uint32_t b = 0; b < 4; ++b) {
-        if ((word >> (8 * b)) & 0xFFu) {
-          s_par[lg] = phi | (K)b;
-          s_fc[lg] = pu_w + (b ? __popc(word & ((1u << (8 * b)) - 1u)) : 0u);
-          ++lg;
+        for (uint32_t b = 0; b < 4; ++b) {
+          if ((word >> (8 * b)) & 0xFFu) {
+            s_fc[lg] = (uint16_t)(pu_w + (b ? __popc(word & ((1u << (8 * b)) - 1u)) : 0u));
+            ++lg;
+          }
+        }
+      } else {
+        const K phi = kl.parent(hi);
+        K* dp = elem2 + (uint64_t)bg + g0;
+        uint32_t* df = first_child2 + (uint64_t)bg + g0;
+#pragma unroll
+        for (uint32_t b = 0; b < 4; ++b) {
+          if ((word >> (8 * b)) & 0xFFu) {
+            dp[lg] = phi | (K)b;
+            df[lg] = bu + pu_w + (b ? __popc(word & ((1u << (8 * b)) - 1u)) : 0u);
+            ++lg;
+          }
         }
       }
     }
@@ -516,22 +532,21 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
           if (i0 + j * BLOCK < nu) dst[i0 + j * BLOCK] = v[j];
       }
     }
-    {
-      // ng <= 4 * BLOCK: one step
+    if (stage) {
+      // ng <= 4 * BLOCK: one step; a group's parent is that of its first key
       const uint32_t i0 = threadIdx.x;
-      K p[4];
       uint32_t f[4];
+      K key[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        p[j] = s_par[i0 + j * BLOCK];
-        f[j] = s_fc[i0 + j * BLOCK];
-      }
+      for (int j = 0; j < 4; ++j) f[j] = s_fc[i0 + j * BLOCK];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) key[j] = s_key[(f[j] - u0) & (EMIT_CAP - 1)];
       K* dp = elem2 + (uint64_t)bg + g0;
       uint32_t* df = first_child2 + (uint64_t)bg + g0;
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         if (i0 + j * BLOCK < ng) {
-          dp[i0 + j * BLOCK] = p[j];
+          dp[i0 + j * BLOCK] = kl.parent(key[j]);
           df[i0 + j * BLOCK] = bu + f[j];
         }
     }
