@@ -377,22 +377,24 @@ __device__ inline uint32_t mesh_of_word(const uint64_t* bm_off, uint32_t n_meshe
   return lo;
 }
 
-// One tile of 2048 words.  The tile's nonzero words are compacted into LDS
-// in word order with the exclusive prefixes of their set bits (low 17 bits)
-// and nonzero bytes (high 15 bits).  Then, 256 nonzero words per round, a
-// lane takes one word and walks its set bits (unique keys, in order) and its
-// nonzero bytes (groups) into LDS buffers, which the block writes out
+// One tile of 2048 words, which lies inside one mesh's bitmap region (the
+// regions start on tile boundaries).  The tile's nonzero words are compacted
+// into LDS in word order with the exclusive prefixes of their set bits (low
+// 17 bits) and nonzero bytes (high 15 bits).  Then, 256 nonzero words per
+// round, a lane takes one word and walks its set bits (unique keys, in order)
+// and its nonzero bytes (groups) into LDS buffers, which the block writes out
 // contiguously.  A lane per nonzero word keeps most lanes busy (a lane per
 // bit would leave about four in five idle at surface densities), and staging
-// turns the lanes' strided stores into whole cache lines.  A group is staged
-// as the 16-bit tile index of its first key only; its parent key is read
-// back from that key at copy-out.  That keeps the workgroup's LDS near 30 KB
-// with 32-bit keys, so five workgroups fit a CU where four did with the
-// parents staged.  A round with more than EMIT_CAP keys (dense tiles) stores
-// its keys and groups directly.  The tile's words are zeroed after they are
-// read.
+// turns the lanes' strided stores into whole cache lines.  A key is staged
+// as its 16-bit bit index in the tile (the tile's first key is added at
+// copy-out) and a group as the index of its first key only (its parent is
+// read back from that key), so the workgroup's LDS stays near 26 KB for
+// either key width and six workgroups fit a CU.  A round with more than
+// EMIT_CAP keys (dense tiles) stores its keys and groups directly.  The
+// tile's words are zeroed after they are read.
 constexpr uint32_t EMIT_CAP = 2048;
 static_assert((EMIT_CAP & (EMIT_CAP - 1)) == 0, "the copy-out masks slot indices");
+static_assert(BM_TILE == BITMAP_TILE_WORDS, "mesh regions start on tile boundaries");
 
 template <typename K>
 __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_words,
@@ -402,11 +404,8 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
                                                    uint32_t* first_child2) {
   __shared__ uint32_t tmp[8];
   __shared__ uint32_t s_word[BM_TILE], s_pre[BM_TILE];
-  __shared__ uint16_t s_idx[BM_TILE];
-  __shared__ K s_key[EMIT_CAP];
-  __shared__ uint16_t s_fc[BLOCK * 4];
-  __shared__ uint32_t s_mesh;
-  __shared__ uint64_t s_moff[2];  // word range [s_moff[0], s_moff[1]) of mesh s_mesh
+  __shared__ uint16_t s_idx[BM_TILE], s_key[EMIT_CAP], s_fc[BLOCK * 4];
+  __shared__ K s_kbase;  // key of the tile's bit 0
   const uint64_t tw0 = (uint64_t)blockIdx.x * BM_TILE;
   const uint32_t l0 = threadIdx.x * BM_WPT;
   // the bitmap words and the tile's output offsets are loaded first, so that
@@ -416,9 +415,7 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
   const uint32_t bu = tile_u[blockIdx.x], bg = tile_g[blockIdx.x];
   if (threadIdx.x == 0) {
     const uint32_t m = mesh_of_word(bm_off, n_meshes, tw0);
-    s_mesh = m;
-    s_moff[0] = bm_off[m];
-    s_moff[1] = bm_off[m + 1];
+    s_kbase = ((K)m << kl.cb) | ((K)(tw0 - bm_off[m]) << 5);
   }
   clear_words(bits, n_words, tw0 + l0, w);  // the bitmap is left clear for the next pass
   // a: set bits (0-16) and nonzero words (17-28); b: nonzero bytes
@@ -447,8 +444,7 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
   __syncthreads();
   const uint32_t nz = ta >> 17;
   const uint32_t pre_end = (ta & 0x1FFFFu) | (tb << 17);
-  const uint32_t m0 = s_mesh;
-  const uint64_t moff0 = s_moff[0], moff1 = s_moff[1];
+  const K kbase = s_kbase;
   for (uint32_t r0 = 0; r0 < nz; r0 += BLOCK) {
     const uint32_t r1 = r0 + BLOCK < nz ? r0 + BLOCK : nz;
     const uint32_t pre0 = s_pre[r0], pre1 = r1 < nz ? s_pre[r1] : pre_end;
@@ -458,41 +454,22 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
     const uint32_t it = r0 + threadIdx.x;
     if (it < r1) {
       const uint32_t word = s_word[it], pre = s_pre[it];
-      // high part of the word's keys: mesh and word offset within the mesh's
-      // region (the tile's first mesh unless the word lies past it; the walk
-      // keeps the region start it loaded, so no per-lane load is waited on
-      // behind this round's stores)
-      const uint64_t gw = tw0 + s_idx[it];
-      uint32_t m = m0;
-      uint64_t off = moff0;
-      if (gw >= moff1) {
-        m = m0 + 1;
-        off = moff1;
-        while (m + 1 < n_meshes) {
-          const uint64_t next = bm_off[m + 1];
-          if (gw < next) break;
-          ++m;
-          off = next;
-        }
-      }
-      const K hi = ((K)m << kl.cb) | ((K)(gw - off) << 5);
+      const uint32_t wb = (uint32_t)s_idx[it] << 5;  // tile bit index of the word's bit 0
       const uint32_t lu = (pre & 0x1FFFFu) - u0;
-      // unique keys: the set bits in order (two loops, so that each store
-      // names its address space)
+      // unique keys: the set bits in order
       if (stage) {
         uint32_t o = lu;
-        for (uint32_t r = word; r; r &= r - 1u) s_key[o++] = hi | (K)__builtin_ctz(r);
+        for (uint32_t r = word; r; r &= r - 1u) s_key[o++] = (uint16_t)(wb | __builtin_ctz(r));
       } else {
         K* out = elem1 + (uint64_t)bu + u0 + lu;
+        const K hi = kbase + (K)wb;
         for (uint32_t r = word; r; r &= r - 1u) *out++ = hi | (K)__builtin_ctz(r);
       }
-      // groups: the nonzero bytes in order; first_child2 is the unique key of
-      // the byte's lowest set bit
-      // the four bytes are tested in turn (a loop over the nonzero bytes runs
-      // as often as the wave's fullest word has them, nearly always four);
-      // the parent of byte b's keys is parent(hi) + b, and s_fc holds the
-      // tile-relative index (the set bits below the byte's lowest are those
-      // of the lower bytes)
+      // groups: the nonzero bytes in order, tested in turn (a loop over the
+      // nonzero bytes runs as often as the wave's fullest word has them,
+      // nearly always four).  A group's first child is the unique key of
+      // the byte's lowest set bit (the set bits below it are those of the
+      // lower bytes); its parent is parent(word's key 0) + byte index.
       uint32_t lg = (pre >> 17) - g0;
       const uint32_t pu_w = pre & 0x1FFFFu;
       if (stage) {
@@ -504,7 +481,7 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
           }
         }
       } else {
-        const K phi = kl.parent(hi);
+        const K phi = kl.parent(kbase + (K)wb);
         K* dp = elem2 + (uint64_t)bg + g0;
         uint32_t* df = first_child2 + (uint64_t)bg + g0;
 #pragma unroll
@@ -524,29 +501,26 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
     if (stage) {
       K* dst = elem1 + (uint64_t)bu + u0;
       for (uint32_t i0 = threadIdx.x; i0 < nu; i0 += 4 * BLOCK) {
-        K v[4];
+        uint32_t v[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) v[j] = s_key[(i0 + j * BLOCK) & (EMIT_CAP - 1)];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          if (i0 + j * BLOCK < nu) dst[i0 + j * BLOCK] = v[j];
+          if (i0 + j * BLOCK < nu) dst[i0 + j * BLOCK] = kbase + (K)v[j];
       }
-    }
-    if (stage) {
-      // ng <= 4 * BLOCK: one step; a group's parent is that of its first key
+      // groups, ng <= 4 * BLOCK: one step
       const uint32_t i0 = threadIdx.x;
-      uint32_t f[4];
-      K key[4];
+      uint32_t f[4], v[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) f[j] = s_fc[i0 + j * BLOCK];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) key[j] = s_key[(f[j] - u0) & (EMIT_CAP - 1)];
+      for (int j = 0; j < 4; ++j) v[j] = s_key[(f[j] - u0) & (EMIT_CAP - 1)];
       K* dp = elem2 + (uint64_t)bg + g0;
       uint32_t* df = first_child2 + (uint64_t)bg + g0;
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         if (i0 + j * BLOCK < ng) {
-          dp[i0 + j * BLOCK] = kl.parent(key[j]);
+          dp[i0 + j * BLOCK] = kl.parent(kbase + (K)v[j]);
           df[i0 + j * BLOCK] = bu + f[j];
         }
     }

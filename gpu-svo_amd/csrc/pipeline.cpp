@@ -439,9 +439,9 @@ double pass_bytes_bound(const svo_params* p, int kb, uint64_t P, uint64_t T, int
       attrs ? (16.0 * 8 + 28.0 * 12 + (p->semantics == SVO_SEM_INTENDED ? 28.0 * 6 : 0)) : 0.0;
   const double out = (K + 161.0 + rec) * N + K * U[1] + 12 * 256.0;
   // workspace
-  const uint64_t bm_words =
-      ((uint64_t)morton3<uint64_t>((uint32_t)p->res[0] - 1, (uint32_t)p->res[1] - 1,
-                                   (uint32_t)p->res[2] - 1) + 1 + 31) / 32;
+  const uint64_t bm_words = bitmap_region_words(
+      (uint64_t)morton3<uint64_t>((uint32_t)p->res[0] - 1, (uint32_t)p->res[1] - 1,
+                                  (uint32_t)p->res[2] - 1) + 1);
   const double key_cap = std::max(cells, 1.0);
   const uint64_t kc = (uint64_t)std::min(key_cap, 4.0e18);
   double lv = 0.0, cap = key_cap;
@@ -961,7 +961,8 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
       d.small_max = (uint32_t)std::min<uint64_t>(d.small_max, std::max<uint64_t>(32, T / 128));
 
   // occupancy-bitmap words per mesh: one bit per Morton code below
-  // morton(res - 1) + 1 (the reference's dense array size, SVO.cu:342-345)
+  // morton(res - 1) + 1 (the reference's dense array size, SVO.cu:342-345),
+  // each mesh's region rounded up to whole tiles of the level-1 pass
   std::vector<uint64_t> bm_off(n + 1, 0);
   for (uint32_t i = 0; i < n; ++i) {
     const MeshDev& d = md[i];
@@ -969,7 +970,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
     if (d.res[0] > 0 && d.res[1] > 0 && d.res[2] > 0)
       g = (uint64_t)morton3<uint64_t>((uint32_t)d.res[0] - 1, (uint32_t)d.res[1] - 1,
                                       (uint32_t)d.res[2] - 1) + 1;
-    bm_off[i + 1] = bm_off[i] + (g + 31) / 32;
+    bm_off[i + 1] = bm_off[i] + bitmap_region_words(g);
   }
   const uint64_t bm_words = bm_off[n];
 

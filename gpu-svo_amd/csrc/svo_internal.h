@@ -100,12 +100,19 @@ void launch_level_first(const K* sorted, const unsigned long long* n_sorted, uin
                         unsigned long long* counts, LevelSpan* spans1, uint32_t n_levels,
                         uint32_t blocks, void* tmp, hipStream_t s);
 // Level 1 from the occupancy bitmap (n_words words; mesh m owns words
-// [bm_off[m], bm_off[m+1])): the same outputs as launch_level_first followed
+// [bm_off[m], bm_off[m+1]), and every bm_off[m] is a multiple of
+// BITMAP_TILE_WORDS): the same outputs as launch_level_first followed
 // by launch_level_first_finalize, and the bitmap is left all zero (the bytes
 // that takes are added to *reset_bytes).  `tmp` needs
 // bitmap_level_tmp_bytes(n_words); `lvl_tmp` (level_tmp_bytes of l1.cap) is
 // prepared for the later levels.
 size_t bitmap_level_tmp_bytes(uint64_t n_words);
+// Words per tile of the bitmap's level-1 pass; each mesh's bitmap region
+// starts on a tile boundary, so that no tile holds two meshes' words.
+constexpr uint64_t BITMAP_TILE_WORDS = 2048;
+inline uint64_t bitmap_region_words(uint64_t codes) {
+  return ((codes + 31) / 32 + BITMAP_TILE_WORDS - 1) / BITMAP_TILE_WORDS * BITMAP_TILE_WORDS;
+}
 template <typename K>
 void launch_level_first_bitmap(uint32_t* bits, uint64_t n_words, const uint64_t* bm_off,
                                uint32_t n_meshes, KeyLayout<K> kl, LevelArrays l1,
