@@ -401,7 +401,11 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
                                                    const uint64_t* bm_off, uint32_t n_meshes,
                                                    KeyLayout<K> kl, const uint32_t* tile_u,
                                                    const uint32_t* tile_g, K* elem1, K* elem2,
-                                                   uint32_t* first_child2) {
+                                                   uint32_t* first_child2,
+                                                   const unsigned long long* counts,
+                                                   LevelSpan* spans, uint32_t n_levels,
+                                                   MeshTreeState* state, uint64_t* status,
+                                                   uint32_t* tickets) {
   __shared__ uint32_t tmp[8];
   __shared__ uint32_t s_word[BM_TILE], s_pre[BM_TILE];
   __shared__ uint16_t s_idx[BM_TILE], s_key[EMIT_CAP], s_fc[BLOCK * 4];
@@ -415,7 +419,35 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
   const uint32_t bu = tile_u[blockIdx.x], bg = tile_g[blockIdx.x];
   if (threadIdx.x == 0) {
     const uint32_t m = mesh_of_word(bm_off, n_meshes, tw0);
-    s_kbase = ((K)m << kl.cb) | ((K)(tw0 - bm_off[m]) << 5);
+    const uint64_t off = bm_off[m];
+    s_kbase = ((K)m << kl.cb) | ((K)(tw0 - off) << 5);
+    if (tw0 == off) {
+      // the mesh's first tile: its level-1 span ends where the next mesh's
+      // first tile starts (every region is whole tiles) and, for a mesh whose
+      // level 1 holds a single voxel, its depth-1 root (what
+      // k_first_finalize does on the sort path)
+      const uint64_t tn = bm_off[m + 1] / BM_TILE;
+      const uint32_t ue = tn < gridDim.x ? tile_u[tn] : (uint32_t)counts[0];
+      const uint32_t ge = tn < gridDim.x ? tile_g[tn] : (uint32_t)counts[1];
+      LevelSpan& sp = spans[(uint64_t)m * n_levels];
+      sp.elem_first = bu;
+      sp.elem_end = ue;
+      sp.group_first = bg;
+      sp.group_end = ge;
+      if (ue - bu == 1) {
+        state[m].depth = 1;
+        state[m].root_elem = bu;
+      }
+    }
+  }
+  {
+    // the later levels' look-back words and tickets start at zero (as
+    // k_level_clear leaves them), a slice per workgroup
+    const uint64_t words = 2 * ((counts[0] + TILE - 1) / TILE);
+    const uint64_t per = (words + gridDim.x - 1) / gridDim.x;
+    const uint64_t i1 = std::min<uint64_t>(words, (uint64_t)(blockIdx.x + 1) * per);
+    for (uint64_t i = (uint64_t)blockIdx.x * per + threadIdx.x; i < i1; i += BLOCK) status[i] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < MAX_TICKETS) tickets[threadIdx.x] = 0;
   }
   clear_words(bits, n_words, tw0 + l0, w);  // the bitmap is left clear for the next pass
   // a: set bits (0-16) and nonzero words (17-28); b: nonzero bytes
@@ -525,72 +557,6 @@ __global__ __launch_bounds__(BLOCK) void k_bm_emit(uint32_t* bits, uint64_t n_wo
         }
     }
     __syncthreads();
-  }
-}
-
-// Level-1 spans of each mesh: its ranges in the sorted unique keys and in the
-// groups (both sorted by mesh).  One wave per (mesh, bound): each step probes
-// 64 evenly spaced positions and keeps the interval where the predicate
-// "mesh of the key < target" turns false.
-template <typename K>
-__device__ inline uint32_t wave_first_of_mesh(const K* a, uint64_t n, KeyLayout<K> kl,
-                                              uint32_t target) {
-  const uint32_t lane = __lane_id();
-  uint64_t lo = 0, hi = n;  // answer in [lo, hi]
-  while (hi - lo > 64) {
-    const uint64_t step = (hi - lo + 63) / 64;
-    const uint64_t p = lo + lane * step;
-    const bool valid = p < hi;
-    const bool lt = valid && kl.mesh(a[p]) < target;
-    const uint32_t c = (uint32_t)__popcll(__ballot(lt));  // probes 0 .. c-1 are below
-    const uint64_t nlo = c == 0 ? lo : lo + (uint64_t)(c - 1) * step + 1;
-    const uint64_t cand = lo + (uint64_t)c * step;
-    const uint64_t nhi = cand < hi ? cand : hi;
-    lo = nlo;
-    hi = nhi;
-  }
-  const uint64_t p = lo + lane;
-  const bool lt = p < hi && kl.mesh(a[p]) < target;
-  return (uint32_t)(lo + (uint64_t)__popcll(__ballot(lt)));
-}
-
-// Blocks 0 .. n_meshes-1: level-1 spans of mesh m (one wave per bound) and,
-// for a mesh whose level 1 holds a single voxel, its depth-1 root (what
-// k_first_finalize does on the sort path).  Every block: zeroes the later
-// levels' look-back words and tickets, as k_level_clear does.
-template <typename K>
-__global__ __launch_bounds__(BLOCK) void k_bm_finish(const K* elem1, const K* elem2,
-                                                     const unsigned long long* counts,
-                                                     KeyLayout<K> kl, uint32_t n_meshes,
-                                                     LevelSpan* spans, uint32_t n_levels,
-                                                     MeshTreeState* state, uint64_t* status,
-                                                     uint32_t* tickets) {
-  __shared__ uint32_t s_r[4];
-  {
-    const uint64_t words = 2 * ((counts[0] + TILE - 1) / TILE);
-    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
-    for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; i < words; i += stride)
-      status[i] = 0;
-    if (blockIdx.x == 0 && threadIdx.x < MAX_TICKETS) tickets[threadIdx.x] = 0;
-  }
-  const uint32_t m = blockIdx.x;
-  if (m >= n_meshes) return;  // uniform over the workgroup
-  const uint32_t w = threadIdx.x >> 6;  // 0: elem_first 1: elem_end 2: group_first 3: group_end
-  const bool groups = w >= 2;
-  const uint32_t r = wave_first_of_mesh(groups ? elem2 : elem1, groups ? counts[1] : counts[0],
-                                        kl, m + (w & 1));
-  if (__lane_id() == 0) s_r[w] = r;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    LevelSpan& sp = spans[(uint64_t)m * n_levels];
-    sp.elem_first = s_r[0];
-    sp.elem_end = s_r[1];
-    sp.group_first = s_r[2];
-    sp.group_end = s_r[3];
-    if (s_r[1] - s_r[0] == 1) {  // a single voxel: depth-1 tree
-      state[m].depth = 1;
-      state[m].root_elem = s_r[0];
-    }
   }
 }
 
@@ -783,19 +749,15 @@ void launch_level_first_bitmap(uint32_t* bits, uint64_t n_words, const uint64_t*
                      tile_g, tile_r, scan_status, n_status);
   hipLaunchKernelGGL(k_bm_scan, dim3((uint32_t)chunks), dim3(BLOCK), 0, s, tile_u, tile_g,
                      (const uint32_t*)tile_r, (uint32_t)tiles, scan_status, counts, reset_bytes);
-  hipLaunchKernelGGL(k_bm_emit<K>, dim3((uint32_t)tiles), dim3(BLOCK), 0, s, bits, n_words,
-                     bm_off, n_meshes, kl, (const uint32_t*)tile_u, (const uint32_t*)tile_g,
-                     (K*)l1.elem, (K*)l2.elem, l2.first_child);
-  // spans, depth-1 roots, and the later levels' look-back words and tickets
-  // (launch_level)
+  // the emission also writes the level-1 spans and depth-1 roots, and clears
+  // the later levels' look-back words and tickets (launch_level)
   uint32_t* tickets = (uint32_t*)lvl_tmp;
   uint64_t* status = (uint64_t*)((char*)lvl_tmp + 256);
-  const uint64_t ltiles = (std::max<uint64_t>(l1.cap, 1) + TILE - 1) / TILE;
-  const uint32_t cg = (uint32_t)std::min<uint64_t>(2048, (2 * ltiles + BLOCK - 1) / BLOCK);
-  hipLaunchKernelGGL(k_bm_finish<K>, dim3(std::max<uint32_t>(std::max<uint32_t>(cg, 1), n_meshes)),
-                     dim3(BLOCK), 0, s, (const K*)l1.elem, (const K*)l2.elem,
-                     (const unsigned long long*)counts, kl, n_meshes, spans1, n_levels, state,
-                     status, tickets);
+  hipLaunchKernelGGL(k_bm_emit<K>, dim3((uint32_t)tiles), dim3(BLOCK), 0, s, bits, n_words,
+                     bm_off, n_meshes, kl, (const uint32_t*)tile_u, (const uint32_t*)tile_g,
+                     (K*)l1.elem, (K*)l2.elem, l2.first_child,
+                     (const unsigned long long*)counts, spans1, n_levels, state, status,
+                     tickets);
 }
 
 void launch_level_first_finalize(const LevelSpan* spans, uint32_t n_meshes, uint32_t n_levels,
