@@ -811,11 +811,18 @@ struct StageTimer {
   std::vector<hipEvent_t> side_ev;
   bool off = false;  // no events (a pass under graph capture)
   StageTimer(Context& c, Lane& l, bool disabled = false) : C(c), L(l), off(disabled) {}
+  // Timing-only events: no system-scope fence when they complete (a fence
+  // writes back and invalidates the caches at every stage boundary, which
+  // lengthens the stage after it); the host reads them only after the pass's
+  // streams are synchronized.
+  static hipError_t create_event(hipEvent_t* e) {
+    return hipEventCreateWithFlags(e, hipEventDisableSystemFence);
+  }
   void mark(const char* name) {
     if (!C.timing || off) return;
     if (L.events.size() <= used) {
       hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return;
+      if (create_event(&e) != hipSuccess) return;
       L.events.push_back(e);
     }
     hipEventRecord(L.events[used], L.stream);
@@ -830,7 +837,7 @@ struct StageTimer {
     const size_t slot = 64 + i;
     while (L.events.size() <= slot) {
       hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) return 0;
+      if (create_event(&e) != hipSuccess) return 0;
       L.events.push_back(e);
     }
     hipEventRecord(L.events[slot], st);
@@ -1827,7 +1834,7 @@ int run_batch(Context& C, const std::vector<MeshInput>& in, std::vector<svo_tree
   CallTiming ct;
   if (C.timing) {
     hipSetDevice(C.device);
-    if (hipEventCreate(&ct.ref) == hipSuccess) {
+    if (StageTimer::create_event(&ct.ref) == hipSuccess) {
       // on the null stream, which the lanes' non-blocking streams do not
       // wait for and nothing else uses: it completes when the call begins
       if (hipEventRecord(ct.ref, nullptr) != hipSuccess) {
