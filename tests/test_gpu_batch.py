@@ -47,6 +47,39 @@ def test_batch_equals_plain_and_oracle(gpu_ctx, sem, path):
         t.free()
 
 
+def test_batch_noncubic_grids(gpu_ctx):
+    """Bitmap regions whose word counts are not whole tiles (non-cubic and
+    small grids) are padded to tile boundaries; the level-1 spans and
+    depth-1 roots then come from the tiles' offsets.  A batch of such meshes,
+    with a one-voxel mesh among them, must equal the plain builds and the
+    oracle."""
+    from svo_amd import meshgen
+    specs = [sp for sp in meshgen.mixed_specs(16, seed=41, res_choices=(64,))
+             if sp.n_tris <= 20480]  # keep the oracle cheap
+    grids = [(17, 90, 33), (64, 32, 48), (5, 7, 3), (100, 37, 64), (1, 1, 1), (33, 33, 33),
+             (8, 120, 9), (64, 64, 65), (3, 3, 200), (31, 17, 29), (2, 2, 2), (70, 20, 90)]
+    meshes = [sp.build() + (g,) for sp, g in zip(specs, grids)]
+    assert len(meshes) == len(grids)
+    print(f"batch of {len(meshes)} meshes on non-cubic grids", flush=True)
+    gpu_ctx.set_option("dedup_path", 1)
+    try:
+        trees, st = gpu_ctx.build_batch(meshes)
+    finally:
+        gpu_ctx.set_option("dedup_path", 0)
+    assert all(s == 0 for s in st), st
+    try:
+        for i, ((P, T, res), tb) in enumerate(zip(meshes, trees)):
+            tp = gpu_ctx.build(P, T, res)
+            assert tb.depth == tp.depth, (i, res)
+            for f in ALL:
+                assert_same(f"mesh {i} {res} {f}", tb.field(f), tp.field(f))
+            compare(tb, po.build(P, T, res))
+            tp.free()
+    finally:
+        for t in trees:
+            t.free()
+
+
 def test_batch_device_inputs(gpu_ctx):
     meshes = _meshes(24, seed=22, res_choices=(64, 128, 256))
     dms = [gpu_ctx.upload_mesh(P, T) for P, T, _ in meshes]
