@@ -275,35 +275,35 @@ __global__ __launch_bounds__(BLOCK) void k_bm_count(const uint32_t* bits, uint64
                                                     uint32_t* tile_u, uint32_t* tile_g,
                                                     uint32_t* tile_r, uint64_t* scan_status,
                                                     uint32_t n_status) {
-  __shared__ uint64_t tmp[4];
+  __shared__ uint32_t tmp[8];
   // the tile-count scan's look-back words and ticket start at zero
   if (blockIdx.x == 0)
     for (uint32_t i = threadIdx.x; i < n_status; i += BLOCK) scan_status[i] = 0;
   const uint64_t w0 = (uint64_t)blockIdx.x * BM_TILE + threadIdx.x * BM_WPT;
   uint32_t w[BM_WPT];
   load_words(bits, n_words, w0, w);
-  // packed: set bits in 0-23, nonzero bytes in 24-43, 4-byte units zeroed by
-  // clear_words in 44-63
-  uint64_t cnt = 0;
+  // a: set bits (0-16) and the 4-byte units clear_words will zero (17-28);
+  // b: nonzero bytes
+  uint32_t a = 0, b = 0;
 #pragma unroll
-  for (int j = 0; j < BM_WPT; ++j)
-    cnt += (uint64_t)__popc(w[j]) | ((uint64_t)nonzero_bytes(w[j]) << 24);
+  for (int j = 0; j < BM_WPT; ++j) {
+    a += __popc(w[j]);
+    b += nonzero_bytes(w[j]);
+  }
   if (w0 + BM_WPT <= n_words) {
-    uint32_t units = 0;
 #pragma unroll
     for (int q = 0; q < BM_WPT / 4; ++q)
-      units += (w[4 * q] | w[4 * q + 1] | w[4 * q + 2] | w[4 * q + 3]) ? 4u : 0u;
-    cnt += (uint64_t)units << 44;
+      a += (w[4 * q] | w[4 * q + 1] | w[4 * q + 2] | w[4 * q + 3]) ? (4u << 17) : 0u;
   } else {
 #pragma unroll
-    for (int j = 0; j < BM_WPT; ++j) cnt += (uint64_t)(w[j] != 0) << 44;
+    for (int j = 0; j < BM_WPT; ++j) a += (uint32_t)(w[j] != 0) << 17;
   }
-  uint64_t tot;
-  block_excl_scan256(cnt, tmp, tot);
+  uint32_t ta, tb;
+  block_excl_scan256_2(a, b, tmp, ta, tb);  // only the totals are used
   if (threadIdx.x == 0) {
-    tile_u[blockIdx.x] = (uint32_t)(tot & 0xFFFFFFu);
-    tile_g[blockIdx.x] = (uint32_t)((tot >> 24) & 0xFFFFFu);
-    tile_r[blockIdx.x] = (uint32_t)(tot >> 44);
+    tile_u[blockIdx.x] = ta & 0x1FFFFu;
+    tile_g[blockIdx.x] = tb;
+    tile_r[blockIdx.x] = ta >> 17;
   }
 }
 
