@@ -182,6 +182,26 @@ __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
     bit |= __shfl_xor(bit, 1);
     bit |= __shfl_xor(bit, 2);
     bit |= __shfl_xor(bit, 4);
+    if (out.fill_neighbors) {
+      // the node's 27 neighbor slots, all NONE (the roots' slots and the
+      // intended semantics' neighbors are written by later kernels); when
+      // the wave's 64 nodes are live and consecutive, as 27 coalesced stores
+      // over the wave's 1728 slots
+      const uint64_t tn = q.nbase + q.off_l + 8ull * (q.g - q.sp.group_first) + k;
+      const uint64_t tb = tn - __lane_id();
+      const uint64_t tb0 =
+          ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(tb >> 32)) << 32) |
+          (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)tb);
+      if (__all(live && tb == tb0)) {
+        uint32_t* nb = out.neighbors + 27 * tb0 + __lane_id();
+#pragma unroll
+        for (int i = 0; i < 27; ++i) nb[64 * i] = NONE;
+      } else if (live) {
+        uint32_t* nb = out.neighbors + 27 * tn;
+#pragma unroll
+        for (int i = 0; i < 27; ++i) nb[i] = NONE;
+      }
+    }
     if (!live) continue;
     const uint32_t mask = bit;
     const bool real = (mask >> k) & 1u;
@@ -238,8 +258,14 @@ __global__ void k_emit_roots(KeyLayout<K> kl, LevelPtrs lp, uint32_t n_meshes,
   out.width[t] = w;
   out.parent[t] = NONE;
   store_childs(out.childs + 8 * t, d >= 2 ? (uint32_t)L.level_off[d - 2] : 0u, d >= 2);
-  // findNeighbors<true> (SVO.cu:528-529): root.neighbors[13] = root index
-  if (d >= 2) out.neighbors[27 * t + 13] = (uint32_t)local;
+  // findNeighbors<true> (SVO.cu:528-529): root.neighbors[13] = root index,
+  // every other slot NONE (written here when no fill kernel ran)
+  if (out.fill_neighbors) {
+    for (int i = 0; i < 27; ++i)
+      out.neighbors[27 * t + i] = (i == 13 && d >= 2) ? (uint32_t)local : NONE;
+  } else if (d >= 2) {
+    out.neighbors[27 * t + 13] = (uint32_t)local;
+  }
   (void)spans;
   (void)n_levels;
 }

@@ -1309,6 +1309,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   na.childs = (uint32_t*)(ob + o_ch);
   na.neighbors = (uint32_t*)(ob + o_nb);
   na.code_bytes = cbytes;
+  na.fill_neighbors = 0;
   na.skip = capture ? rec->d_flag : nullptr;
   out->voxels = ob + o_vx;
   out->verts = (svo_vertex*)(ob + o_v);
@@ -1336,7 +1337,9 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
   // write stream) runs on the side stream beside the node records (bound by
   // their loads' latency); the roots' neighbor slot is written after the
   // join.  Concurrent passes already overlap each other, and their extra
-  // streams would outnumber the hardware queues, so they fill in order.
+  // streams would outnumber the hardware queues: there the node and root
+  // kernels write the neighbor slots themselves (wave-coalesced stores
+  // beside their record writes).
   bool side_fill = C.active_passes.load() == 1 && !capture;
   if (side_fill && (hipEventRecord(L.fork, s) != hipSuccess ||
                     hipStreamWaitEvent(L.side, L.fork, 0) != hipSuccess)) {
@@ -1354,8 +1357,7 @@ int run_typed(Context& C, Lane& L, const std::vector<MeshInput>& in, const Plan&
       side_fill = false;
     }
   } else {
-    launch_fill_u32(na.neighbors, NONE, 27 * N, s);
-    timer.mark("neighbor_fill");
+    na.fill_neighbors = 1;
   }
   for (uint32_t l = 1; l < NL; ++l)
     launch_emit_level_nodes<K>((int)l, kl, lv[l].elem, h_counts[l - 1], lv[l + 1].elem,

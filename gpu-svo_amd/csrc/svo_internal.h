@@ -151,6 +151,9 @@ struct NodeArrays {
   uint32_t* childs;      // 8 per node
   uint32_t* neighbors;   // 27 per node
   int code_bytes;
+  // Nonzero: the node and root kernels also set each node's 27 neighbor
+  // slots to NONE; zero: a fill kernel does that before them.
+  int fill_neighbors;
   // Graph-captured passes: when *skip is nonzero (a size differs from the
   // recorded run) the node, neighbor and record kernels return at once, so
   // that nothing is written past the recorded layout.  Null otherwise.
