@@ -377,6 +377,8 @@ def _roofline(stats, stages):
     dev = {k: v for k, v in stages.items() if not k.startswith("setup") and k != "start"}
     dom = max(dev, key=dev.get)
     b = stage_bytes(stats, dom)
+    if dom == "nodes" and "neighbor_fill" not in stages:
+        b += stage_bytes(stats, "neighbor_fill")  # concurrent passes: the node kernels fill
     ach = b / (stages[dom] * 1e-3) / 1e9 if stages[dom] > 0 else 0.0
     ratio = TRAFFIC_PER_ALG_BYTE.get(dom)
     return {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
