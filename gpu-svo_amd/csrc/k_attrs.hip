@@ -170,23 +170,16 @@ __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
     const uint64_t j = (uint64_t)a[u].fc + k;
     a[u].e = j < n_elem_l ? elem_l[j] : (K)0;
   }
+  if (out.fill_neighbors) {
+    // the nodes' 27 neighbor slots, all NONE (the roots' slots and the
+    // intended semantics' neighbors are written by later kernels), issued
+    // while the child keys are still loading: they need the mesh records
+    // only.  When the wave's 64 nodes are live and consecutive, as 27
+    // coalesced stores over the wave's 1728 slots.
 #pragma unroll
-  for (int u = 0; u < NODES_PT; ++u) {
-    const NodeIn<K>& q = a[u];
-    const uint64_t j = (uint64_t)q.fc + k;
-    // a depth-1 tree's level-1 group is no node group; a mesh rejected after
-    // the level phase has no nodes
-    const bool live = q.in && q.depth > (uint32_t)level && q.nn != 0;
-    uint32_t bit =
-        (live && j < n_elem_l && kl.parent(q.e) == q.P) ? 1u << (uint32_t)(q.e & 7) : 0u;
-    bit |= __shfl_xor(bit, 1);
-    bit |= __shfl_xor(bit, 2);
-    bit |= __shfl_xor(bit, 4);
-    if (out.fill_neighbors) {
-      // the node's 27 neighbor slots, all NONE (the roots' slots and the
-      // intended semantics' neighbors are written by later kernels); when
-      // the wave's 64 nodes are live and consecutive, as 27 coalesced stores
-      // over the wave's 1728 slots
+    for (int u = 0; u < NODES_PT; ++u) {
+      const NodeIn<K>& q = a[u];
+      const bool live = q.in && q.depth > (uint32_t)level && q.nn != 0;
       const uint64_t tn = q.nbase + q.off_l + 8ull * (q.g - q.sp.group_first) + k;
       const uint64_t tb = tn - __lane_id();
       const uint64_t tb0 =
@@ -202,6 +195,19 @@ __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
         for (int i = 0; i < 27; ++i) nb[i] = NONE;
       }
     }
+  }
+#pragma unroll
+  for (int u = 0; u < NODES_PT; ++u) {
+    const NodeIn<K>& q = a[u];
+    const uint64_t j = (uint64_t)q.fc + k;
+    // a depth-1 tree's level-1 group is no node group; a mesh rejected after
+    // the level phase has no nodes
+    const bool live = q.in && q.depth > (uint32_t)level && q.nn != 0;
+    uint32_t bit =
+        (live && j < n_elem_l && kl.parent(q.e) == q.P) ? 1u << (uint32_t)(q.e & 7) : 0u;
+    bit |= __shfl_xor(bit, 1);
+    bit |= __shfl_xor(bit, 2);
+    bit |= __shfl_xor(bit, 4);
     if (!live) continue;
     const uint32_t mask = bit;
     const bool real = (mask >> k) & 1u;
