@@ -186,9 +186,20 @@ __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
           ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(tb >> 32)) << 32) |
           (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)tb);
       if (__all(live && tb == tb0)) {
-        uint32_t* nb = out.neighbors + 27 * tb0 + __lane_id();
+        // 1728 slots from a 4-byte boundary: up to 3 single slots to reach
+        // a 16-byte boundary, 16-byte stores, then the remaining slots
+        uint32_t* nb = out.neighbors + 27 * tb0;
+        const uint32_t head = (uint32_t)((4 - (((uintptr_t)nb >> 2) & 3)) & 3);
+        const uint32_t lane = __lane_id();
+        if (lane < head) nb[lane] = NONE;
+        uint4* body = (uint4*)(nb + head);
+        const uint32_t n4 = (1728 - head) >> 2;
+        const uint4 none4 = make_uint4(NONE, NONE, NONE, NONE);
 #pragma unroll
-        for (int i = 0; i < 27; ++i) nb[64 * i] = NONE;
+        for (uint32_t i = 0; i < 7; ++i)
+          if (lane + 64 * i < n4) body[lane + 64 * i] = none4;
+        const uint32_t tail = (1728 - head) & 3;
+        if (lane < tail) nb[head + 4 * n4 + lane] = NONE;
       } else if (live) {
         uint32_t* nb = out.neighbors + 27 * tn;
 #pragma unroll
