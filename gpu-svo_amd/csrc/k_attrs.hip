@@ -174,8 +174,8 @@ __global__ __launch_bounds__(BLOCK) void k_emit_nodes(
     // the nodes' 27 neighbor slots, all NONE (the roots' slots and the
     // intended semantics' neighbors are written by later kernels), issued
     // while the child keys are still loading: they need the mesh records
-    // only.  When the wave's 64 nodes are live and consecutive, as 27
-    // coalesced stores over the wave's 1728 slots.
+    // only.  When the wave's 64 nodes are live and consecutive, their 1728
+    // slots are one contiguous run, written by the wave with 16-byte stores.
 #pragma unroll
     for (int u = 0; u < NODES_PT; ++u) {
       const NodeIn<K>& q = a[u];
